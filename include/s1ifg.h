@@ -1,0 +1,236 @@
+/*
+ * s1ifg.h — C ABI of the MI355X interferogram-formation library (libs1ifg.so).
+ *
+ * The reference (WWSAR/sentinel_gpu_processor) has no in-process interface
+ * for this path: Python launches standalone executables and exchanges argv,
+ * stdin/stdout, files and an SQLite parameter table with them.  Each entry
+ * point below replaces one stage of those executables; the comment on each
+ * names the reference code it replaces (file:line in the reference tree).
+ *
+ * Conventions
+ *  - Every pointer to pixel data is a DEVICE pointer (hipMalloc / torch
+ *    tensor storage) unless the comment says host.  Small parameter blocks
+ *    (orbit, burst metadata, grids, polynomials) are HOST pointers.
+ *  - The caller allocates every input and output.  A context owns only its
+ *    stream binding and a workspace it grows on demand.
+ *  - Return value: 0 on success, a negative S1IFG_E* code on failure; the
+ *    message is available from s1ifg_last_error() on the calling thread.
+ *    Nothing in the library exits or aborts the process.
+ *  - Every entry point enqueues work on the context stream and returns
+ *    without synchronising; s1ifg_ctx_sync() waits.
+ *  - Complex data are interleaved float pairs (re, im), row-major.
+ *  - One context per host thread (or per stream); contexts are independent.
+ */
+#ifndef S1IFG_H
+#define S1IFG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define S1IFG_ABI_VERSION 1
+
+enum {
+  S1IFG_OK = 0,
+  S1IFG_EINVAL = -1, /* bad argument (sizes, null pointers, metadata) */
+  S1IFG_EHIP = -2,   /* HIP runtime error */
+  S1IFG_ENOMEM = -3, /* device allocation failed */
+  S1IFG_ERANGE = -4, /* size beyond what the kernels support */
+  S1IFG_ESTATE = -5  /* object used with the wrong context / device */
+};
+
+enum { S1IFG_INTERP_BILINEAR = 0, S1IFG_INTERP_SINC = 1 };
+enum { S1IFG_GOLD_FP32 = 0, S1IFG_GOLD_FP16_SPLIT = 1 };
+enum { S1IFG_DEM_F32 = 0, S1IFG_DEM_I16 = 1 };
+/* coherence flags */
+enum {
+  S1IFG_COH_FILL_CLAMP = 0, /* gamma > 1 -> 1 */
+  S1IFG_COH_FILL_NAN = 1,   /* gamma > 1 -> NaN (mask) */
+  S1IFG_COH_FILL_HASH = 2   /* gamma > 1 -> seeded uniform [0,1) */
+};
+
+typedef struct { float re, im; } s1ifg_cf32;
+
+typedef struct s1ifg_ctx s1ifg_ctx;
+typedef struct s1ifg_geom s1ifg_geom;
+
+/* Orbit state vectors, uniformly spaced in time (HOST pointers).
+ * Replaces the orbit text file read by read_orbit_ascii (orbit.cu:147-168). */
+typedef struct {
+  int32_t n;          /* number of state vectors, >= 4 */
+  int32_t pad_;
+  const double *t;    /* [n] seconds */
+  const double *pos;  /* [n*3] ECEF metres */
+  const double *vel;  /* [n*3] ECEF metres/second */
+} s1ifg_orbit;
+
+/* Burst metadata.  Replaces the SQLite `file` table keys read at
+ * geo2rdr.cu:333-376 plus the FM-rate / Doppler-centroid files read by
+ * read_polynomials (sario.cu:80-103).  fm/dc are HOST arrays of records
+ * (t, t0, c0, c1, c2). */
+typedef struct {
+  int32_t lines;               /* linesPerBurst */
+  int32_t samples;             /* samplesPerBurst */
+  int32_t first_valid_line;    /* firstValidLine{k} */
+  int32_t last_valid_line;     /* lastValidLine{k} */
+  int32_t first_valid_sample;  /* firstValidSample{k} */
+  int32_t last_valid_sample;   /* lastValidSample{k} */
+  double t_start;              /* azimuthTimeSeconds{k} */
+  double prf;                  /* prf (reproject uses 1/prf) */
+  double azimuth_time_interval;/* azimuthTimeInterval (deramp eta) */
+  double slant_range_time;     /* slantRangeTime, two-way seconds */
+  double range_sampling_rate;  /* rangeSamplingRate, Hz */
+  double wavelength;           /* wvl, metres */
+  double radar_frequency;      /* radarFrequency, Hz */
+  double azimuth_steering_rate;/* azimuthSteeringRate, deg/s */
+  int32_t n_fm;
+  int32_t n_dc;
+  const double *fm;            /* [n_fm*5] */
+  const double *dc;            /* [n_dc*5] */
+} s1ifg_burst;
+
+/* Target lat/lon grid (the .rsc keys Y_FIRST, X_FIRST, Y_STEP, X_STEP,
+ * FILE_LENGTH, WIDTH; sario.hpp:399-408).  Pixel (r, c) is at
+ * (lat0 + r*dlat, lon0 + c*dlon). */
+typedef struct {
+  int32_t nrow, ncol;
+  double lat0, lon0, dlat, dlon;
+} s1ifg_grid;
+
+/* DEM on its own lat/lon grid (DEVICE data); heights are bilinearly
+ * interpolated at target pixels, with indices clamped to the DEM.  When the
+ * DEM grid equals the target grid this is the reference's h = dem[i]
+ * (geo2rdr.cu:199). */
+typedef struct {
+  const void *data;
+  int32_t dtype;  /* S1IFG_DEM_F32 or S1IFG_DEM_I16 */
+  int32_t nrow, ncol;
+  int32_t pad_;
+  double lat0, lon0, dlat, dlon;
+} s1ifg_dem;
+
+/* ---------------------------------------------------------------- context */
+int s1ifg_abi_version(void);
+/* Thread-local message of the last failing call on this thread. */
+const char *s1ifg_last_error(void);
+/* Replaces set_gpu / parse_gpu_arg (gpu_device.hpp:22-79). */
+int s1ifg_ctx_create(int device, s1ifg_ctx **out);
+void s1ifg_ctx_destroy(s1ifg_ctx *ctx);
+/* Bind a HIP stream (hipStream_t as void*); NULL = the null stream. */
+int s1ifg_ctx_set_stream(s1ifg_ctx *ctx, void *stream);
+int s1ifg_ctx_sync(s1ifg_ctx *ctx);
+/* Workspace currently held by the context, in bytes. */
+int s1ifg_ctx_workspace(s1ifg_ctx *ctx, int64_t *bytes);
+
+/* ----------------------------------------------------- burst geometry */
+/* Host FP64 set-up of one burst: orbit interpolation polynomials, ks, FM/DC
+ * coefficients at the burst centre, kt/etaref per range sample, the burst
+ * mid-time orbit state.  Replaces geo2rdr.cu:386-400 and 465-549 and the
+ * device copies at 444-461, 545-549, 596-600. */
+int s1ifg_geom_create(s1ifg_ctx *ctx, const s1ifg_orbit *orbit,
+                      const s1ifg_burst *burst, s1ifg_geom **out);
+void s1ifg_geom_destroy(s1ifg_geom *g);
+/* Scalars of the set-up for inspection: out[0]=ks, out[1..4]=fm (t0,c0,c1,c2),
+ * out[5..8]=dc (t0,c0,c1,c2), out[9]=etac0, out[10]=tmid, out[11..13]=xmid,
+ * out[14..16]=vmid.  n >= 17. */
+int s1ifg_geom_query(const s1ifg_geom *g, double *out, int n);
+/* Burst footprint (latmin, latmax, lonmin, lonmax) for heights in
+ * [hmin, hmax]; replaces bounds()/rah2ll (bounds.cu:237-317). HOST output. */
+int s1ifg_geom_bounds(const s1ifg_geom *g, double hmin, double hmax,
+                      double latlon_out[4]);
+/* Orbit position/velocity/acceleration at n times (DEVICE arrays):
+ * replaces intp_orbit (orbit.cu:224-245) for validation. */
+int s1ifg_orbit_eval(s1ifg_ctx *ctx, const s1ifg_geom *g, const double *t,
+                     int64_t n, double *pos, double *vel, double *acc);
+
+/* --------------------------------------------------------- burst stages */
+/* A8 deramp: out = in * exp(-i*pi*kt[j]*(eta[i]-etaref[j])^2); in may equal
+ * out.  Replaces deramp_burst (geo2rdr.cu:138-158). */
+int s1ifg_deramp(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_cf32 *in,
+                 s1ifg_cf32 *out);
+
+/* A5/A9 geo2rdr offsets on the target grid (validation mode: the fused
+ * kernels never materialise them).  dh: optional per-pixel DEM error (float,
+ * DEVICE) added to the DEM height.  offpoly: optional HOST double[6] offset
+ * polynomial (see s1ifg_geocode).  rgoff/azoff/rng: DEVICE double
+ * [nrow*ncol]; any may be NULL.  Replaces the Newton part of reproject
+ * (geo2rdr.cu:195-207; orbitrangetime orbit.cu:247-279). */
+int s1ifg_geo2rdr(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_grid *grid,
+                  const s1ifg_dem *dem, const float *dh, const double *offpoly,
+                  double *rgoff, double *azoff, double *rng);
+
+/* A9 geocode one deramped burst onto the target grid: geo2rdr, validity
+ * window, interpolation (bilinear as the reference, or 8-tap windowed sinc),
+ * reramp and range flattening; invalid pixels are 0.  offpoly (HOST double[6],
+ * nullable) shifts the sampling position by
+ *   d_rg = p0 + p1*(rgoff/samples) + p2*(azoff/lines)
+ *   d_az = p3 + p4*(rgoff/samples) + p5*(azoff/lines)  pixels.
+ * Replaces reproject (geo2rdr.cu:170-248). */
+int s1ifg_geocode(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_grid *grid,
+                  const s1ifg_dem *dem, const float *dh, const double *offpoly,
+                  const s1ifg_cf32 *deramped, int interp, s1ifg_cf32 *out);
+
+/* A10 first and last-exclusive non-zero rows of an image into DEVICE
+ * int32[2] ({0,0} when all zero).  Replaces
+ * find_first_last_nonzero_rows_kernel + find_nonzero_rows
+ * (geo2rdr.cu:86-128, 273-299). */
+int s1ifg_valid_rows(s1ifg_ctx *ctx, const s1ifg_cf32 *img, int nrow, int ncol,
+                     int32_t *first_last);
+
+/* ------------------------------------------------------ pair products */
+/* A12/A13/A16 fused crossmul + look means: ifg_ml = mean over rl x cl of
+ * conj(ref)*sec; pow_ref/pow_sec (nullable) = mean |ref|^2, |sec|^2 on the
+ * same cells.  Strides in elements.  Output is (nrow/rl) x (ncol/cl) with row
+ * stride ncol/cl.  Replaces conj_mul, cpx_col_look, cpx_row_look
+ * (sario.cu:199-286) as launched by launch_async_pipeline
+ * (crossmul_daemon.cu:921-975). */
+int s1ifg_crossmul_looks(s1ifg_ctx *ctx, const s1ifg_cf32 *ref, int64_t ld_ref,
+                         const s1ifg_cf32 *sec, int64_t ld_sec, int nrow,
+                         int ncol, int rl, int cl, s1ifg_cf32 *ifg_ml,
+                         float *pow_ref, float *pow_sec);
+
+/* A16 amplitude looks of one image: amp = sqrt(mean over rl x cl of |s|^2);
+ * at rl = cl = 1 the reference returns sqrt(|s|) (sario.py:416-418) and so
+ * does this call.  Replaces multilook/powlooks (coherence.py:14-55,
+ * sario.py:397-420). */
+int s1ifg_amplitude(s1ifg_ctx *ctx, const s1ifg_cf32 *slc, int64_t ld,
+                    int nrow, int ncol, int rl, int cl, float *amp);
+
+/* A17 coherence on the multilooked grid.  win = 1: the reference look-cell
+ * estimator |ifg|/(amp1+1e-12)/(amp2+1e-12) with amp = sqrt(pow)
+ * (coherence.py:142-144; the reference's unseeded random fill for gamma > 1
+ * is replaced by the S1IFG_COH_FILL_* rule).  win odd >= 3: windowed
+ * estimator |sum ifg| / sqrt(sum pow_ref * sum pow_sec) over win x win,
+ * clipped at the borders (no reference counterpart). */
+int s1ifg_coherence(s1ifg_ctx *ctx, const s1ifg_cf32 *ifg_ml,
+                    const float *pow_ref, const float *pow_sec, int nrow,
+                    int ncol, int win, int fill, uint32_t seed, float *coh);
+
+/* A18 Goldstein adaptive filter (patch n_win with step n_win/4, alpha), with
+ * a deterministic gather-form overlap-add.  mode: S1IFG_GOLD_FP32 (radix-2
+ * FFT in FP32) or S1IFG_GOLD_FP16_SPLIT (DFT-by-GEMM on MFMA, hi/lo split
+ * FP16 operands, FP32 accumulation).  Pixels whose weight sum is 0 are 0.
+ * n_win must be 32; nrow, ncol >= n_win.  Replaces goldstein_filter_cuda_host
+ * and its kernels (goldstein.cu:87-408). */
+int s1ifg_goldstein(s1ifg_ctx *ctx, const s1ifg_cf32 *in, int nrow, int ncol,
+                    int n_win, float alpha, int mode, s1ifg_cf32 *out);
+
+/* Fused burst pair: geo2rdr for both orbits, interpolation of both deramped
+ * bursts, reramp + flattening, crossmul and look means, in one pass over the
+ * target grid (the geocoded SLCs are never written).  offpoly applies to the
+ * secondary.  Outputs as s1ifg_crossmul_looks over the whole target grid.
+ * Replaces the geo2rdr -> .gslc -> crossmul_daemon chain
+ * (geo2rdr.cu:170-248, crossmul_daemon.cu:921-975). */
+int s1ifg_pair(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
+               const s1ifg_grid *grid, const s1ifg_dem *dem, const float *dh,
+               const double *offpoly_sec, const s1ifg_cf32 *ref_deramped,
+               const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl,
+               s1ifg_cf32 *ifg_ml, float *pow_ref, float *pow_sec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* S1IFG_H */

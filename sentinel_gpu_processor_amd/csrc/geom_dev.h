@@ -1,0 +1,236 @@
+// Device functions of the geometry path (shared by the per-stage and fused
+// kernels so that both produce the same bits for the same pixel).
+#pragma once
+
+#include "s1ifg_internal.h"
+
+namespace s1ifg {
+
+// Per-stencil record in the orbit buffer: [t_k, coef[24], pad...].
+constexpr int kOrbRec = 32;
+
+// Position, velocity and acceleration from the polynomial form of the
+// 4-node Hermite interpolation (orbit.cu:170-245).  Stencil index: truncated
+// (t - t0)/(t1 - t0) clamped to [1, n-3] (orbit.cu:229-232).
+__device__ __forceinline__ void orbit_eval(const OrbitDev &o, double t, double x[3],
+                                           double v[3], double a[3]) {
+  int k = (int)((t - o.t0) / o.h);
+  k = k > 1 ? k : 1;
+  k = k < o.n - 3 ? k : o.n - 3;
+  const double *rec = o.coef + (size_t)(k - 1) * kOrbRec;
+  const double inv_h = 1.0 / o.h;
+  const double s = (t - rec[0]) * inv_h;
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    const double *c = rec + 1 + comp * 8;
+    double p = c[7], dp = 0.0, ddp = 0.0;
+#pragma unroll
+    for (int m = 6; m >= 0; --m) {
+      ddp = fma(ddp, s, 2.0 * dp);
+      dp = fma(dp, s, p);
+      p = fma(p, s, c[m]);
+    }
+    x[comp] = p;
+    v[comp] = dp * inv_h;
+    a[comp] = ddp * inv_h * inv_h;
+  }
+}
+
+// Zero-Doppler azimuth time of X (orbitrangetime, orbit.cu:247-279).  The
+// iteration starts from the burst mid-time state as the reference does
+// (geo2rdr.cu:588-600).  The Newton derivative includes the dr.a term, so
+// the iteration converges quadratically to the same root; after the step
+// that meets the 5e-9 s criterion the orbit is advanced by a second-order
+// Taylor step instead of a further interpolation.  Returns dr = X - x(t).
+__device__ __forceinline__ int zero_doppler(const GeomDev &g, const double X[3], double &t,
+                                            double dr[3]) {
+  double x[3] = {g.xmid[0], g.xmid[1], g.xmid[2]};
+  double v[3] = {g.vmid[0], g.vmid[1], g.vmid[2]};
+  double a[3] = {g.amid[0], g.amid[1], g.amid[2]};
+  t = g.tmid;
+  for (int it = 0; it < 51; ++it) {
+    double d0 = X[0] - x[0], d1 = X[1] - x[1], d2 = X[2] - x[2];
+    double fn = d0 * v[0] + d1 * v[1] + d2 * v[2];
+    double fp = -(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + (d0 * a[0] + d1 * a[1] + d2 * a[2]);
+    double dt = -fn / fp;
+    t += dt;
+    if (fabs(dt) < 5.0e-9) {
+      double h2 = 0.5 * dt * dt;
+      dr[0] = d0 - v[0] * dt - a[0] * h2;
+      dr[1] = d1 - v[1] * dt - a[1] * h2;
+      dr[2] = d2 - v[2] * dt - a[2] * h2;
+      return it + 1;
+    }
+    orbit_eval(g.orb, t, x, v, a);
+  }
+  dr[0] = X[0] - x[0];
+  dr[1] = X[1] - x[1];
+  dr[2] = X[2] - x[2];
+  return 52;
+}
+
+// Row / column tables of the target grid (FP64): see k_grid_tables.
+struct RowTab { double coslat, sinlat, re, fy; };
+struct ColTab { double coslon, sinlon, fx; };
+
+__device__ __forceinline__ double dem_sample(const DemDev &d, int iy, int ix) {
+  size_t o = (size_t)iy * d.ncol + ix;
+  if (d.dtype == S1IFG_DEM_I16) return (double)((const short *)d.data)[o];
+  return (double)((const float *)d.data)[o];
+}
+
+// Bilinear DEM height with indices clamped to the DEM.
+__device__ __forceinline__ double dem_height(const DemDev &d, double fy, double fx) {
+  fy = fmin(fmax(fy, 0.0), (double)(d.nrow - 1));
+  fx = fmin(fmax(fx, 0.0), (double)(d.ncol - 1));
+  int iy = min((int)floor(fy), d.nrow - 2);
+  int ix = min((int)floor(fx), d.ncol - 2);
+  double wy = fy - iy, wx = fx - ix;
+  double h00 = dem_sample(d, iy, ix), h01 = dem_sample(d, iy, ix + 1);
+  double h10 = dem_sample(d, iy + 1, ix), h11 = dem_sample(d, iy + 1, ix + 1);
+  return (h00 * (1 - wx) + h01 * wx) * (1 - wy) + (h10 * (1 - wx) + h11 * wx) * wy;
+}
+
+// llh2xyz (orbit.cu:15-35) with the per-row / per-column trigonometry
+// precomputed.
+__device__ __forceinline__ void target_xyz(const RowTab &r, const ColTab &c, double h,
+                                           double X[3]) {
+  double rh = (r.re + h) * r.coslat;
+  X[0] = rh * c.coslon;
+  X[1] = rh * c.sinlon;
+  X[2] = (r.re * (1 - kRe2) + h) * r.sinlat;
+}
+
+// Reduce a phase to [-pi, pi] in FP64 and return it as float.
+__device__ __forceinline__ float reduce_phase(double ph) {
+  double k = rint(ph * (1.0 / kTwoPi));
+  return (float)fma(-k, kTwoPi, ph);
+}
+
+// 8-tap Hann-windowed sinc weights for fractional offset f in [0,1):
+// w_k ~ sinc(f - k) * cos^2(pi (f - k)/8), k = -3..4, normalised to sum 1.
+__device__ __forceinline__ void sinc_weights(float f, float w[8]) {
+  float s, c, s4, c4;
+  sincospif(f, &s, &c);
+  sincospif(0.25f * f, &s4, &c4);
+  // cos(pi k/4), sin(pi k/4) for k = -3..4
+  const float ck[8] = {-0.70710678f, 0.0f, 0.70710678f, 1.0f, 0.70710678f, 0.0f, -0.70710678f, -1.0f};
+  const float sk[8] = {-0.70710678f, -1.0f, -0.70710678f, 0.0f, 0.70710678f, 1.0f, 0.70710678f, 0.0f};
+  float sum = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    int k = i - 3;
+    float x = f - (float)k;
+    // sin(pi (f - k)) = (-1)^k sin(pi f)
+    float sn = (k & 1) ? -s : s;
+    float sc = (x == 0.0f) ? 1.0f : sn / (3.14159265358979f * x);
+    // cos(pi (f-k)/4) = cos(pi f/4) cos(pi k/4) + sin(pi f/4) sin(pi k/4)
+    float cw = c4 * ck[i] + s4 * sk[i];
+    float wv = sc * 0.5f * (1.0f + cw);
+    w[i] = wv;
+    sum += wv;
+  }
+  float inv = 1.0f / sum;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] *= inv;
+}
+
+// Geocode one target pixel of one burst (reproject, geo2rdr.cu:194-247).
+// Returns the complex value (0 outside the validity window).  rgoff/azoff
+// are the sampling positions (offset polynomial applied), rng the geometric
+// range.
+template <int INTERP>
+__device__ __forceinline__ float2 geocode_pixel(const GeomDev &g, const OffPoly &op,
+                                                const double X[3], const float2 *__restrict__ z,
+                                                double *rg_out = nullptr,
+                                                double *az_out = nullptr,
+                                                double *rng_out = nullptr) {
+  double t, dr[3];
+  zero_doppler(g, X, t, dr);
+  double rng = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+  double rgoff = (rng - g.rngstart) / g.dmrg;
+  double azoff = (t - g.tstart) / g.dtaz;
+  double drg = 0.0, daz = 0.0;
+  if (op.on) {
+    double xs = rgoff / g.samples, ys = azoff / g.lines;
+    drg = op.p[0] + op.p[1] * xs + op.p[2] * ys;
+    daz = op.p[3] + op.p[4] * xs + op.p[5] * ys;
+  }
+  double rg = rgoff + drg, az = azoff + daz;
+  if (rg_out) *rg_out = rg;
+  if (az_out) *az_out = az;
+  if (rng_out) *rng_out = rng;
+  bool valid;
+  if (INTERP == S1IFG_INTERP_BILINEAR)
+    valid = rg > g.fvs && rg < g.lvs - 1 && az >= g.flv + 5 && az < g.llv - 5;
+  else
+    valid = rg >= g.fvs + 3 && rg < g.lvs - 4 && az >= g.flv + 5 && az < g.llv - 5;
+  if (!valid) return make_float2(0.f, 0.f);
+  // reramp (geo2rdr.cu:210-219) at the sampling position
+  double rp = rng + drg * g.dmrg;
+  double dt = rp * 2 / kSol - g.fm[0];
+  double ka = g.fm[1] + dt * (g.fm[2] + g.fm[3] * dt);
+  dt = rp * 2 / kSol - g.dc[0];
+  double fnc = g.dc[1] + dt * (g.dc[2] + g.dc[3] * dt);
+  double kt = ka * g.ks / (ka - g.ks);
+  double etac = -fnc / ka;
+  double etaref = etac - g.etac0;
+  double eta = -g.lines * g.dtaz / 2. + (t - g.tstart) + daz * g.dtaz;
+  double etadiff = eta - etaref;
+  double reramp = -kPi * kt * etadiff * etadiff;
+  float ph = reduce_phase(4.0 * kPi / g.wvl * rng - reramp);
+  float sp, cp;
+  sincosf(ph, &sp, &cp);
+  float rx, ry;
+  const int N = g.samples;
+  if (INTERP == S1IFG_INTERP_BILINEAR) {
+    // geo2rdr.cu:220-237: range blends rounded to float, azimuth blend FP64
+    int intr = (int)rg;
+    double fr = rg - intr;
+    int inta = (int)az;
+    double fa = az - inta;
+    size_t i1 = (size_t)inta * N + intr;
+    float2 b1 = z[i1], b2 = z[i1 + 1], b3 = z[i1 + N], b4 = z[i1 + N + 1];
+    float c1x = (float)(b1.x * (1 - fr) + b2.x * fr);
+    float c1y = (float)(b1.y * (1 - fr) + b2.y * fr);
+    float c2x = (float)(b3.x * (1 - fr) + b4.x * fr);
+    float c2y = (float)(b3.y * (1 - fr) + b4.y * fr);
+    rx = (float)(c1x * (1 - fa) + c2x * fa);
+    ry = (float)(c1y * (1 - fa) + c2y * fa);
+  } else {
+    int intr = (int)floor(rg);
+    int inta = (int)floor(az);
+    float wr[8], wa[8];
+    sinc_weights((float)(rg - intr), wr);
+    sinc_weights((float)(az - inta), wa);
+    rx = 0.f;
+    ry = 0.f;
+    const float2 *base = z + (size_t)(inta - 3) * N + (intr - 3);
+#pragma unroll
+    for (int ia = 0; ia < 8; ++ia) {
+      const float2 *row = base + (size_t)ia * N;
+      float ax = 0.f, ay = 0.f;
+#pragma unroll
+      for (int ir = 0; ir < 8; ++ir) {
+        float2 v = row[ir];
+        ax = fmaf(wr[ir], v.x, ax);
+        ay = fmaf(wr[ir], v.y, ay);
+      }
+      rx = fmaf(wa[ia], ax, rx);
+      ry = fmaf(wa[ia], ay, ry);
+    }
+  }
+  return make_float2(rx * cp - ry * sp, rx * sp + ry * cp);
+}
+
+__device__ __forceinline__ void pixel_xyz(const GridDev &gr, const DemDev &dem,
+                                          const RowTab *rows, const ColTab *cols,
+                                          const float *dh, int r, int c, double X[3]) {
+  RowTab rt = rows[r];
+  ColTab ct = cols[c];
+  double h = dem_height(dem, rt.fy, ct.fx);
+  if (dh) h += (double)dh[(size_t)r * gr.ncol + c];
+  target_xyz(rt, ct, h, X);
+}
+
+}  // namespace s1ifg

@@ -1,0 +1,353 @@
+// Geometry-path kernels: grid tables, orbit evaluation, deramp, geo2rdr,
+// geocode, valid-row scan, and the fused burst-pair kernel.
+#include "geom_dev.h"
+
+using namespace s1ifg;
+
+namespace {
+
+// Row / column tables of the target grid: trigonometry of llh2xyz
+// (orbit.cu:15-35) and the DEM sampling coordinates.
+__global__ void k_grid_tables(GridDev gr, DemDev dem, RowTab *rows, ColTab *cols) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < gr.nrow) {
+    double lat = gr.lat0 + i * gr.dlat;
+    double rl = lat * kPi / 180;
+    double sl, cl;
+    sincos(rl, &sl, &cl);
+    RowTab r;
+    r.coslat = cl;
+    r.sinlat = sl;
+    r.re = kRa / sqrt(1. - kRe2 * sl * sl);
+    r.fy = (lat - dem.lat0) / dem.dlat;
+    rows[i] = r;
+  }
+  if (i < gr.ncol) {
+    double lon = gr.lon0 + i * gr.dlon;
+    double sn, cs;
+    sincos(lon * kPi / 180, &sn, &cs);
+    ColTab c;
+    c.coslon = cs;
+    c.sinlon = sn;
+    c.fx = (lon - dem.lon0) / dem.dlon;
+    cols[i] = c;
+  }
+}
+
+__global__ void k_orbit_eval(OrbitDev o, const double *t, long n, double *pos, double *vel,
+                             double *acc) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x[3], v[3], a[3];
+  orbit_eval(o, t[i], x, v, a);
+  for (int k = 0; k < 3; ++k) {
+    if (pos) pos[i * 3 + k] = x[k];
+    if (vel) vel[i * 3 + k] = v[k];
+    if (acc) acc[i * 3 + k] = a[k];
+  }
+}
+
+// deramp_burst (geo2rdr.cu:138-158): phase in FP64 (eta as geo2rdr.cu:529-532),
+// reduced to [-pi, pi] before a float sincos.  Two pixels per thread.
+// in and out may alias (in-place deramp): no __restrict__ here.
+__global__ void k_deramp(GeomDev g, const float4 *in2, float4 *out2, const float2 *in1,
+                         float2 *out1, int vec) {
+  const int i = blockIdx.y;
+  const int N = g.samples;
+  const double eta = -g.lines * g.ati / 2. + i * g.ati;
+  const size_t rowoff = (size_t)i * N;
+  for (int jj = blockIdx.x * blockDim.x + threadIdx.x; jj < (N + 1) / 2;
+       jj += gridDim.x * blockDim.x) {
+    int j = 2 * jj;
+    if (vec) {
+      float4 a = in2[(rowoff >> 1) + jj];
+      double d0 = eta - g.etaref[j], d1 = eta - g.etaref[j + 1];
+      float p0 = reduce_phase(-kPi * g.kt[j] * d0 * d0);
+      float p1 = reduce_phase(-kPi * g.kt[j + 1] * d1 * d1);
+      float s0, c0, s1, c1;
+      sincosf(p0, &s0, &c0);
+      sincosf(p1, &s1, &c1);
+      float4 o;
+      o.x = a.x * c0 - a.y * s0;
+      o.y = a.x * s0 + a.y * c0;
+      o.z = a.z * c1 - a.w * s1;
+      o.w = a.z * s1 + a.w * c1;
+      out2[(rowoff >> 1) + jj] = o;
+    } else {
+      for (int k = j; k < j + 2 && k < N; ++k) {
+        float2 a = in1[rowoff + k];
+        double d = eta - g.etaref[k];
+        float p = reduce_phase(-kPi * g.kt[k] * d * d);
+        float s, c;
+        sincosf(p, &s, &c);
+        out1[rowoff + k] = make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+      }
+    }
+  }
+}
+
+__global__ void k_geo2rdr(GeomDev g, OffPoly op, GridDev gr, DemDev dem, const RowTab *rows,
+                          const ColTab *cols, const float *dh, double *rgoff, double *azoff,
+                          double *rng) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (r >= gr.nrow || c >= gr.ncol) return;
+  double X[3];
+  pixel_xyz(gr, dem, rows, cols, dh, r, c, X);
+  double t, dr[3];
+  zero_doppler(g, X, t, dr);
+  double R = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+  double rg = (R - g.rngstart) / g.dmrg;
+  double az = (t - g.tstart) / g.dtaz;
+  if (op.on) {
+    double xs = rg / g.samples, ys = az / g.lines;
+    double drg = op.p[0] + op.p[1] * xs + op.p[2] * ys;
+    double daz = op.p[3] + op.p[4] * xs + op.p[5] * ys;
+    rg += drg;
+    az += daz;
+  }
+  size_t o = (size_t)r * gr.ncol + c;
+  if (rgoff) rgoff[o] = rg;
+  if (azoff) azoff[o] = az;
+  if (rng) rng[o] = R;
+}
+
+template <int INTERP>
+__global__ void k_geocode(GeomDev g, OffPoly op, GridDev gr, DemDev dem, const RowTab *rows,
+                          const ColTab *cols, const float *dh, const float2 *__restrict__ z,
+                          float2 *__restrict__ out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (r >= gr.nrow || c >= gr.ncol) return;
+  double X[3];
+  pixel_xyz(gr, dem, rows, cols, dh, r, c, X);
+  out[(size_t)r * gr.ncol + c] = geocode_pixel<INTERP>(g, op, X, z);
+}
+
+// Fused burst pair: one thread per multilooked pixel; the look cell is
+// accumulated as column means then a row mean (the order of
+// cpx_col_look / cpx_row_look, sario.cu:241-286).
+template <int INTERP>
+__global__ void k_pair(GeomDev gr_, GeomDev gs, OffPoly ops, GridDev gr, DemDev dem,
+                       const RowTab *rows, const ColTab *cols, const float *dh,
+                       const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl,
+                       int cl, int nr_ml, int nc_ml, float2 *__restrict__ ifg,
+                       float *__restrict__ pr, float *__restrict__ ps) {
+  int cm = blockIdx.x * blockDim.x + threadIdx.x;
+  int rm = blockIdx.y * blockDim.y + threadIdx.y;
+  if (rm >= nr_ml || cm >= nc_ml) return;
+  const OffPoly none = {{0, 0, 0, 0, 0, 0}, 0};
+  float ax = 0.f, ay = 0.f, apr = 0.f, aps = 0.f;
+  const float inv_cl = 1.0f / cl, inv_rl = 1.0f / rl;
+  for (int i = 0; i < rl; ++i) {
+    int r = rm * rl + i;
+    float sx = 0.f, sy = 0.f, spr = 0.f, sps = 0.f;
+    for (int j = 0; j < cl; ++j) {
+      int c = cm * cl + j;
+      double X[3];
+      pixel_xyz(gr, dem, rows, cols, dh, r, c, X);
+      float2 a = geocode_pixel<INTERP>(gr_, none, X, zr);
+      float2 b = geocode_pixel<INTERP>(gs, ops, X, zs);
+      sx += a.x * b.x + a.y * b.y;
+      sy += a.x * b.y - a.y * b.x;
+      spr += a.x * a.x + a.y * a.y;
+      sps += b.x * b.x + b.y * b.y;
+    }
+    ax += sx * inv_cl;
+    ay += sy * inv_cl;
+    apr += spr * inv_cl;
+    aps += sps * inv_cl;
+  }
+  size_t o = (size_t)rm * nc_ml + cm;
+  ifg[o] = make_float2(ax * inv_rl, ay * inv_rl);
+  if (pr) pr[o] = apr * inv_rl;
+  if (ps) ps[o] = aps * inv_rl;
+}
+
+__global__ void k_rows_init(int *fl, int nrow) {
+  if (threadIdx.x == 0) {
+    fl[0] = nrow;
+    fl[1] = -1;
+  }
+}
+
+// find_first_last_nonzero_rows_kernel (geo2rdr.cu:86-128): one block per row.
+__global__ void k_rows_scan(const float2 *__restrict__ img, int nrow, int ncol, int *fl) {
+  int row = blockIdx.x;
+  const float2 *p = img + (size_t)row * ncol;
+  int any = 0;
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+    float2 v = p[c];
+    if (v.x != 0.0f || v.y != 0.0f) {
+      any = 1;
+      break;
+    }
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0 && any) {
+    atomicMin(&fl[0], row);
+    atomicMax(&fl[1], row + 1);
+  }
+}
+
+__global__ void k_rows_final(int *fl, int nrow) {
+  if (threadIdx.x == 0 && fl[0] == nrow) {
+    fl[0] = 0;
+    fl[1] = 0;
+  }
+}
+
+int grid_tables(s1ifg_ctx *ctx, const GridDev &gr, const DemDev &dem, RowTab **rows,
+                ColTab **cols) {
+  size_t nb = sizeof(RowTab) * (size_t)gr.nrow + sizeof(ColTab) * (size_t)gr.ncol + 256;
+  void *ws = nullptr;
+  int rc = ctx_workspace(ctx, nb, &ws);
+  if (rc) return rc;
+  *rows = (RowTab *)ws;
+  size_t off = (sizeof(RowTab) * (size_t)gr.nrow + 255) & ~(size_t)255;
+  *cols = (ColTab *)((char *)ws + off);
+  int n = gr.nrow > gr.ncol ? gr.nrow : gr.ncol;
+  hipLaunchKernelGGL(k_grid_tables, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, gr, dem,
+                     *rows, *cols);
+  S1_LAUNCH_CHECK("k_grid_tables");
+  return S1IFG_OK;
+}
+
+int common_setup(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_grid *grid,
+                 const s1ifg_dem *dem, const double *offpoly, GridDev *gr, DemDev *dd,
+                 OffPoly *op) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (g && (rc = check_geom(ctx, g))) return rc;
+  if ((rc = make_grid(grid, gr))) return rc;
+  if ((rc = make_dem(dem, dd))) return rc;
+  if ((rc = make_offpoly(offpoly, op))) return rc;
+  return S1IFG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int s1ifg_orbit_eval(s1ifg_ctx *ctx, const s1ifg_geom *g, const double *t, int64_t n,
+                     double *pos, double *vel, double *acc) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if ((rc = check_geom(ctx, g))) return rc;
+  if (n < 0 || (n > 0 && !t)) return set_error(S1IFG_EINVAL, "bad time array");
+  if (n == 0) return S1IFG_OK;
+  hipLaunchKernelGGL(k_orbit_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                     g->dev.orb, t, (long)n, pos, vel, acc);
+  S1_LAUNCH_CHECK("k_orbit_eval");
+  return S1IFG_OK;
+}
+
+int s1ifg_deramp(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_cf32 *in, s1ifg_cf32 *out) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if ((rc = check_geom(ctx, g))) return rc;
+  if (!in || !out) return set_error(S1IFG_EINVAL, "null burst pointer");
+  const GeomDev &gd = g->dev;
+  int pairs = (gd.samples + 1) / 2;
+  int bx = (pairs + 255) / 256;
+  // float4 path: even row length and 16-byte aligned rows; otherwise float2
+  int vec = (gd.samples % 2 == 0) && (((uintptr_t)in | (uintptr_t)out) % 16 == 0);
+  hipLaunchKernelGGL(k_deramp, dim3(bx, gd.lines), dim3(256), 0, ctx->stream, gd,
+                     (const float4 *)in, (float4 *)out, (const float2 *)in, (float2 *)out, vec);
+  S1_LAUNCH_CHECK("k_deramp");
+  return S1IFG_OK;
+}
+
+int s1ifg_geo2rdr(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_grid *grid,
+                  const s1ifg_dem *dem, const float *dh, const double *offpoly, double *rgoff,
+                  double *azoff, double *rng) {
+  GridDev gr;
+  DemDev dd;
+  OffPoly op;
+  int rc = common_setup(ctx, g, grid, dem, offpoly, &gr, &dd, &op);
+  if (rc) return rc;
+  RowTab *rows;
+  ColTab *cols;
+  if ((rc = grid_tables(ctx, gr, dd, &rows, &cols))) return rc;
+  dim3 blk(64, 4), grd((gr.ncol + 63) / 64, (gr.nrow + 3) / 4);
+  hipLaunchKernelGGL(k_geo2rdr, grd, blk, 0, ctx->stream, g->dev, op, gr, dd, rows, cols, dh,
+                     rgoff, azoff, rng);
+  S1_LAUNCH_CHECK("k_geo2rdr");
+  return S1IFG_OK;
+}
+
+int s1ifg_geocode(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_grid *grid,
+                  const s1ifg_dem *dem, const float *dh, const double *offpoly,
+                  const s1ifg_cf32 *deramped, int interp, s1ifg_cf32 *out) {
+  GridDev gr;
+  DemDev dd;
+  OffPoly op;
+  int rc = common_setup(ctx, g, grid, dem, offpoly, &gr, &dd, &op);
+  if (rc) return rc;
+  if (!deramped || !out) return set_error(S1IFG_EINVAL, "null data pointer");
+  if (interp != S1IFG_INTERP_BILINEAR && interp != S1IFG_INTERP_SINC)
+    return set_error(S1IFG_EINVAL, "interp %d", interp);
+  RowTab *rows;
+  ColTab *cols;
+  if ((rc = grid_tables(ctx, gr, dd, &rows, &cols))) return rc;
+  dim3 blk(64, 4), grd((gr.ncol + 63) / 64, (gr.nrow + 3) / 4);
+  if (interp == S1IFG_INTERP_BILINEAR)
+    hipLaunchKernelGGL(k_geocode<S1IFG_INTERP_BILINEAR>, grd, blk, 0, ctx->stream, g->dev, op, gr,
+                       dd, rows, cols, dh, (const float2 *)deramped, (float2 *)out);
+  else
+    hipLaunchKernelGGL(k_geocode<S1IFG_INTERP_SINC>, grd, blk, 0, ctx->stream, g->dev, op, gr, dd,
+                       rows, cols, dh, (const float2 *)deramped, (float2 *)out);
+  S1_LAUNCH_CHECK("k_geocode");
+  return S1IFG_OK;
+}
+
+int s1ifg_valid_rows(s1ifg_ctx *ctx, const s1ifg_cf32 *img, int nrow, int ncol,
+                     int32_t *first_last) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!img || !first_last || nrow <= 0 || ncol <= 0)
+    return set_error(S1IFG_EINVAL, "bad arguments");
+  hipLaunchKernelGGL(k_rows_init, dim3(1), dim3(64), 0, ctx->stream, first_last, nrow);
+  hipLaunchKernelGGL(k_rows_scan, dim3(nrow), dim3(256), 0, ctx->stream, (const float2 *)img, nrow,
+                     ncol, first_last);
+  hipLaunchKernelGGL(k_rows_final, dim3(1), dim3(64), 0, ctx->stream, first_last, nrow);
+  S1_LAUNCH_CHECK("k_rows_scan");
+  return S1IFG_OK;
+}
+
+int s1ifg_pair(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
+               const s1ifg_grid *grid, const s1ifg_dem *dem, const float *dh,
+               const double *offpoly_sec, const s1ifg_cf32 *ref_deramped,
+               const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl, s1ifg_cf32 *ifg_ml,
+               float *pow_ref, float *pow_sec) {
+  GridDev gr;
+  DemDev dd;
+  OffPoly op;
+  int rc = common_setup(ctx, g_ref, grid, dem, offpoly_sec, &gr, &dd, &op);
+  if (rc) return rc;
+  if ((rc = check_geom(ctx, g_sec))) return rc;
+  if (!ref_deramped || !sec_deramped || !ifg_ml) return set_error(S1IFG_EINVAL, "null data pointer");
+  if (rl < 1 || cl < 1) return set_error(S1IFG_EINVAL, "looks %d x %d", rl, cl);
+  if (interp != S1IFG_INTERP_BILINEAR && interp != S1IFG_INTERP_SINC)
+    return set_error(S1IFG_EINVAL, "interp %d", interp);
+  int nr = gr.nrow / rl, nc = gr.ncol / cl;
+  if (nr < 1 || nc < 1) return set_error(S1IFG_EINVAL, "grid smaller than one look cell");
+  RowTab *rows;
+  ColTab *cols;
+  if ((rc = grid_tables(ctx, gr, dd, &rows, &cols))) return rc;
+  dim3 blk(64, 4), grd((nc + 63) / 64, (nr + 3) / 4);
+  if (interp == S1IFG_INTERP_BILINEAR)
+    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, ctx->stream, g_ref->dev,
+                       g_sec->dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
+                       (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
+                       pow_sec);
+  else
+    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_SINC>, grd, blk, 0, ctx->stream, g_ref->dev,
+                       g_sec->dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
+                       (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
+                       pow_sec);
+  S1_LAUNCH_CHECK("k_pair");
+  return S1IFG_OK;
+}
+
+}  // extern "C"

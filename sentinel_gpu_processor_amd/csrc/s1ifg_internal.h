@@ -1,0 +1,102 @@
+// Internal declarations shared by the HIP translation units of libs1ifg.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/s1ifg.h"
+
+namespace s1ifg {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 6.28318530717958647692;
+constexpr double kSol = 299792458.0;   // geo2rdr.cu:26-28
+constexpr double kRa = 6378137.0;      // orbit.hpp:6-8
+constexpr double kRe2 = 0.00669437999015;  // orbit.hpp:9-11
+
+int set_error(int code, const char *fmt, ...);
+int hip_error(hipError_t e, const char *what);
+
+#define S1_HIP(call)                                        \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return ::s1ifg::hip_error(e_, #call); \
+  } while (0)
+
+#define S1_LAUNCH_CHECK(name)                               \
+  do {                                                      \
+    hipError_t e_ = hipGetLastError();                      \
+    if (e_ != hipSuccess) return ::s1ifg::hip_error(e_, name); \
+  } while (0)
+
+// Orbit in polynomial form: for stencil index k in [1, n-3] (the clamped
+// truncated index of orbit.cu:229-232) the 4-node Hermite interpolant is a
+// degree-7 polynomial in s = (t - t_k)/h, h = t_1 - t_0.  coef layout:
+// [(k-1)*24 + comp*8 + m].
+struct OrbitDev {
+  const double *coef;
+  double t0;   // t_0
+  double h;    // t_1 - t_0
+  int n;       // state vectors
+};
+
+// Everything a kernel needs about one burst, passed by value.
+struct GeomDev {
+  OrbitDev orb;
+  double tstart, dtaz, rngstart, dmrg, wvl, ks, etac0;
+  double fm[4];  // t0, c0, c1, c2
+  double dc[4];
+  double tmid, xmid[3], vmid[3], amid[3];
+  double ati;    // azimuth_time_interval (deramp eta)
+  int lines, samples;
+  int flv, llv, fvs, lvs;
+  const double *kt;      // [samples]
+  const double *etaref;  // [samples]
+};
+
+struct GridDev {
+  int nrow, ncol;
+  double lat0, lon0, dlat, dlon;
+};
+
+struct DemDev {
+  const void *data;
+  int dtype, nrow, ncol;
+  double lat0, lon0, dlat, dlon;
+};
+
+struct OffPoly {
+  double p[6];
+  int on;
+};
+
+}  // namespace s1ifg
+
+struct s1ifg_ctx {
+  int device;
+  hipStream_t stream;
+  void *ws;            // workspace
+  size_t ws_bytes;
+  int64_t ws_limit;    // optional cap (S1IFG_WORKSPACE_LIMIT env), -1 = none
+};
+
+struct s1ifg_geom {
+  int device;
+  s1ifg::GeomDev dev;
+  double *d_buf;       // device: orbit coef | kt | etaref
+  // host copies for bounds()
+  int n_orb;
+  double *h_t, *h_pos, *h_vel;
+};
+
+namespace s1ifg {
+// Workspace of at least `bytes` on ctx (grown with hipMalloc, kept).
+int ctx_workspace(s1ifg_ctx *ctx, size_t bytes, void **out);
+int check_ctx(s1ifg_ctx *ctx);
+int check_geom(s1ifg_ctx *ctx, const s1ifg_geom *g);
+int make_grid(const s1ifg_grid *grid, GridDev *out);
+int make_dem(const s1ifg_dem *dem, DemDev *out);
+int make_offpoly(const double *p, OffPoly *out);
+}  // namespace s1ifg

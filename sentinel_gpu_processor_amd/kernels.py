@@ -1,0 +1,175 @@
+"""Tensor-level wrappers of the libs1ifg entry points.
+
+Every function takes and returns device tensors (complex64 / float32 /
+float64) and enqueues the HIP kernels on the context's stream.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+
+def _new(shape, dtype, like_device):
+    return torch.empty(shape, dtype=dtype, device=like_device)
+
+
+def _need(t, dtype, shape=None, name="tensor"):
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} shape {tuple(t.shape)} != {tuple(shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def deramp(ctx, geom, slc, out=None):
+    """TOPS deramp of one burst (geo2rdr.cu:138-158); out may be slc."""
+    b = geom.burst
+    _need(slc, torch.complex64, (b["lines"], b["samples"]), "burst")
+    if out is None:
+        out = torch.empty_like(slc)
+    N.check(ctx.lib.s1ifg_deramp(ctx.handle, geom.handle, N.ptr(slc), N.ptr(out)))
+    return out
+
+
+def _dem_args(dem_t, dem):
+    return N.make_dem(dem_t, dem)
+
+
+def geo2rdr(ctx, geom, grid, dem_t, dem, dh=None, offpoly=None):
+    """Per-target-pixel (rgoff, azoff, range) in FP64 (validation mode)."""
+    g = N.make_grid(grid)
+    d = _dem_args(dem_t, dem)
+    shape = (grid["nrow"], grid["ncol"])
+    rg = _new(shape, torch.float64, dem_t.device)
+    az = _new(shape, torch.float64, dem_t.device)
+    rng = _new(shape, torch.float64, dem_t.device)
+    if dh is not None:
+        _need(dh, torch.float32, shape, "dh")
+    _keep, pp = N.offpoly_arg(offpoly)
+    N.check(ctx.lib.s1ifg_geo2rdr(ctx.handle, geom.handle, ctypes.byref(g), ctypes.byref(d),
+                                  N.ptr(dh), pp, N.ptr(rg), N.ptr(az), N.ptr(rng)))
+    return rg, az, rng
+
+
+def geocode(ctx, geom, grid, dem_t, dem, deramped, interp="bilinear", dh=None, offpoly=None,
+            out=None):
+    """Geocode a deramped burst onto the target grid (geo2rdr.cu:170-248)."""
+    b = geom.burst
+    _need(deramped, torch.complex64, (b["lines"], b["samples"]), "deramped burst")
+    g = N.make_grid(grid)
+    d = _dem_args(dem_t, dem)
+    shape = (grid["nrow"], grid["ncol"])
+    if out is None:
+        out = _new(shape, torch.complex64, deramped.device)
+    if dh is not None:
+        _need(dh, torch.float32, shape, "dh")
+    _keep, pp = N.offpoly_arg(offpoly)
+    N.check(ctx.lib.s1ifg_geocode(ctx.handle, geom.handle, ctypes.byref(g), ctypes.byref(d),
+                                  N.ptr(dh), pp, N.ptr(deramped), N.INTERP[interp], N.ptr(out)))
+    return out
+
+
+def valid_rows(ctx, img):
+    """(first, last_exclusive) non-zero rows (geo2rdr.cu:86-128, 273-299)."""
+    _need(img, torch.complex64, None, "image")
+    fl = torch.empty(2, dtype=torch.int32, device=img.device)
+    N.check(ctx.lib.s1ifg_valid_rows(ctx.handle, N.ptr(img), img.shape[0], img.shape[1],
+                                     N.ptr(fl)))
+    f = fl.cpu().numpy()
+    return int(f[0]), int(f[1])
+
+
+def crossmul_looks(ctx, ref, sec, rl, cl, with_power=True, out=None):
+    """Fused conj(ref)*sec + look means (sario.cu:199-286)."""
+    _need(ref, torch.complex64, None, "ref")
+    _need(sec, torch.complex64, tuple(ref.shape), "sec")
+    nrow, ncol = ref.shape
+    shape = (nrow // rl, ncol // cl)
+    ifg = out if out is not None else _new(shape, torch.complex64, ref.device)
+    pr = _new(shape, torch.float32, ref.device) if with_power else None
+    ps = _new(shape, torch.float32, ref.device) if with_power else None
+    N.check(ctx.lib.s1ifg_crossmul_looks(ctx.handle, N.ptr(ref), ncol, N.ptr(sec), ncol, nrow,
+                                         ncol, rl, cl, N.ptr(ifg), N.ptr(pr), N.ptr(ps)))
+    return ifg, pr, ps
+
+
+def amplitude(ctx, slc, rl, cl):
+    """sqrt of power looks (coherence.py:46-48; 1x1 quirk sqrt|s|)."""
+    _need(slc, torch.complex64, None, "slc")
+    nrow, ncol = slc.shape
+    amp = _new((nrow // rl, ncol // cl), torch.float32, slc.device)
+    N.check(ctx.lib.s1ifg_amplitude(ctx.handle, N.ptr(slc), ncol, nrow, ncol, rl, cl, N.ptr(amp)))
+    return amp
+
+
+_FILL = {"clamp": N.COH_FILL_CLAMP, "nan": N.COH_FILL_NAN, "hash": N.COH_FILL_HASH}
+
+
+def coherence(ctx, ifg_ml, pow_ref, pow_sec, win=1, fill="clamp", seed=0, out=None):
+    """Look-cell (win=1, coherence.py:142) or windowed coherence."""
+    _need(ifg_ml, torch.complex64, None, "ifg")
+    _need(pow_ref, torch.float32, tuple(ifg_ml.shape), "pow_ref")
+    _need(pow_sec, torch.float32, tuple(ifg_ml.shape), "pow_sec")
+    nr, nc = ifg_ml.shape
+    coh = out if out is not None else _new((nr, nc), torch.float32, ifg_ml.device)
+    N.check(ctx.lib.s1ifg_coherence(ctx.handle, N.ptr(ifg_ml), N.ptr(pow_ref), N.ptr(pow_sec),
+                                    nr, nc, win, _FILL[fill], seed, N.ptr(coh)))
+    return coh
+
+
+def goldstein(ctx, ifg, alpha=0.5, n_win=32, mode="fp32", out=None):
+    """Goldstein filter (goldstein.cu:87-408), deterministic overlap-add."""
+    _need(ifg, torch.complex64, None, "ifg")
+    nr, nc = ifg.shape
+    out = out if out is not None else torch.empty_like(ifg)
+    N.check(ctx.lib.s1ifg_goldstein(ctx.handle, N.ptr(ifg), nr, nc, n_win, float(alpha),
+                                    N.GOLD_MODE[mode], N.ptr(out)))
+    return out
+
+
+def pair(ctx, g_ref, g_sec, grid, dem_t, dem, ref_deramped, sec_deramped, interp="bilinear",
+         rl=1, cl=4, dh=None, offpoly=None, out=None):
+    """Fused burst pair -> (ifg_ml, pow_ref, pow_sec) on the look grid."""
+    g = N.make_grid(grid)
+    d = _dem_args(dem_t, dem)
+    br, bs = g_ref.burst, g_sec.burst
+    _need(ref_deramped, torch.complex64, (br["lines"], br["samples"]), "ref burst")
+    _need(sec_deramped, torch.complex64, (bs["lines"], bs["samples"]), "sec burst")
+    shape = (grid["nrow"] // rl, grid["ncol"] // cl)
+    if dh is not None:
+        _need(dh, torch.float32, (grid["nrow"], grid["ncol"]), "dh")
+    if out is None:
+        out = (_new(shape, torch.complex64, ref_deramped.device),
+               _new(shape, torch.float32, ref_deramped.device),
+               _new(shape, torch.float32, ref_deramped.device))
+    ifg, pr, ps = out
+    _keep, pp = N.offpoly_arg(offpoly)
+    N.check(ctx.lib.s1ifg_pair(ctx.handle, g_ref.handle, g_sec.handle, ctypes.byref(g),
+                               ctypes.byref(d), N.ptr(dh), pp, N.ptr(ref_deramped),
+                               N.ptr(sec_deramped), N.INTERP[interp], rl, cl, N.ptr(ifg),
+                               N.ptr(pr), N.ptr(ps)))
+    return ifg, pr, ps
+
+
+def orbit_eval(ctx, geom, t):
+    """Orbit position/velocity/acceleration at device times t (float64)."""
+    _need(t, torch.float64, None, "t")
+    n = t.numel()
+    pos = _new((n, 3), torch.float64, t.device)
+    vel = _new((n, 3), torch.float64, t.device)
+    acc = _new((n, 3), torch.float64, t.device)
+    N.check(ctx.lib.s1ifg_orbit_eval(ctx.handle, geom.handle, N.ptr(t), n, N.ptr(pos),
+                                     N.ptr(vel), N.ptr(acc)))
+    return pos, vel, acc
+
+
+def to_numpy(t):
+    return t.detach().cpu().numpy()
+
+
+def from_numpy(a, device="cuda:0"):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
