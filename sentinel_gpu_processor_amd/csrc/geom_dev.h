@@ -14,11 +14,11 @@ constexpr int kOrbRec = 32;
 // (t - t0)/(t1 - t0) clamped to [1, n-3] (orbit.cu:229-232).
 __device__ __forceinline__ void orbit_eval(const OrbitDev &o, double t, double x[3],
                                            double v[3], double a[3]) {
-  int k = (int)((t - o.t0) / o.h);
+  int k = (int)((t - o.t0) * o.inv_h);
   k = k > 1 ? k : 1;
   k = k < o.n - 3 ? k : o.n - 3;
   const double *rec = o.coef + (size_t)(k - 1) * kOrbRec;
-  const double inv_h = 1.0 / o.h;
+  const double inv_h = o.inv_h;
   const double s = (t - rec[0]) * inv_h;
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
@@ -34,6 +34,16 @@ __device__ __forceinline__ void orbit_eval(const OrbitDev &o, double t, double x
     v[comp] = dp * inv_h;
     a[comp] = ddp * inv_h * inv_h;
   }
+}
+
+// 1/x in FP64 from the float reciprocal and two Newton steps (relative
+// error ~1e-16 for |x| within float range, x != 0).
+__device__ __forceinline__ double rcp64(double x) {
+  double r = (double)__builtin_amdgcn_rcpf((float)x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
 }
 
 // Zero-Doppler azimuth time of X (orbitrangetime, orbit.cu:247-279).  The
@@ -52,7 +62,7 @@ __device__ __forceinline__ int zero_doppler(const GeomDev &g, const double X[3],
     double d0 = X[0] - x[0], d1 = X[1] - x[1], d2 = X[2] - x[2];
     double fn = d0 * v[0] + d1 * v[1] + d2 * v[2];
     double fp = -(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + (d0 * a[0] + d1 * a[1] + d2 * a[2]);
-    double dt = -fn / fp;
+    double dt = -fn * rcp64(fp);
     t += dt;
     if (fabs(dt) < 5.0e-9) {
       double h2 = 0.5 * dt * dt;
@@ -110,9 +120,10 @@ __device__ __forceinline__ float reduce_phase(double ph) {
 // 8-tap Hann-windowed sinc weights for fractional offset f in [0,1):
 // w_k ~ sinc(f - k) * cos^2(pi (f - k)/8), k = -3..4, normalised to sum 1.
 __device__ __forceinline__ void sinc_weights(float f, float w[8]) {
-  float s, c, s4, c4;
-  sincospif(f, &s, &c);
-  sincospif(0.25f * f, &s4, &c4);
+  // hardware sin/cos take revolutions: sin(pi f) = sin(2 pi (f/2))
+  const float s = __builtin_amdgcn_sinf(0.5f * f);
+  const float s4 = __builtin_amdgcn_sinf(0.125f * f);
+  const float c4 = __builtin_amdgcn_cosf(0.125f * f);
   // cos(pi k/4), sin(pi k/4) for k = -3..4
   const float ck[8] = {-0.70710678f, 0.0f, 0.70710678f, 1.0f, 0.70710678f, 0.0f, -0.70710678f, -1.0f};
   const float sk[8] = {-0.70710678f, -1.0f, -0.70710678f, 0.0f, 0.70710678f, 1.0f, 0.70710678f, 0.0f};
@@ -123,14 +134,14 @@ __device__ __forceinline__ void sinc_weights(float f, float w[8]) {
     float x = f - (float)k;
     // sin(pi (f - k)) = (-1)^k sin(pi f)
     float sn = (k & 1) ? -s : s;
-    float sc = (x == 0.0f) ? 1.0f : sn / (3.14159265358979f * x);
+    float sc = (x == 0.0f) ? 1.0f : sn * __builtin_amdgcn_rcpf(3.14159265358979f * x);
     // cos(pi (f-k)/4) = cos(pi f/4) cos(pi k/4) + sin(pi f/4) sin(pi k/4)
     float cw = c4 * ck[i] + s4 * sk[i];
     float wv = sc * 0.5f * (1.0f + cw);
     w[i] = wv;
     sum += wv;
   }
-  float inv = 1.0f / sum;
+  float inv = __builtin_amdgcn_rcpf(sum);
 #pragma unroll
   for (int i = 0; i < 8; ++i) w[i] *= inv;
 }
@@ -148,11 +159,11 @@ __device__ __forceinline__ float2 geocode_pixel(const GeomDev &g, const OffPoly 
   double t, dr[3];
   zero_doppler(g, X, t, dr);
   double rng = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-  double rgoff = (rng - g.rngstart) / g.dmrg;
-  double azoff = (t - g.tstart) / g.dtaz;
+  double rgoff = (rng - g.rngstart) * g.inv_dmrg;
+  double azoff = (t - g.tstart) * g.inv_dtaz;
   double drg = 0.0, daz = 0.0;
   if (op.on) {
-    double xs = rgoff / g.samples, ys = azoff / g.lines;
+    double xs = rgoff * g.inv_samples, ys = azoff * g.inv_lines;
     drg = op.p[0] + op.p[1] * xs + op.p[2] * ys;
     daz = op.p[3] + op.p[4] * xs + op.p[5] * ys;
   }
@@ -172,8 +183,8 @@ __device__ __forceinline__ float2 geocode_pixel(const GeomDev &g, const OffPoly 
   double ka = g.fm[1] + dt * (g.fm[2] + g.fm[3] * dt);
   dt = rp * 2 / kSol - g.dc[0];
   double fnc = g.dc[1] + dt * (g.dc[2] + g.dc[3] * dt);
-  double kt = ka * g.ks / (ka - g.ks);
-  double etac = -fnc / ka;
+  double kt = ka * g.ks * rcp64(ka - g.ks);
+  double etac = -fnc * rcp64(ka);
   double etaref = etac - g.etac0;
   double eta = -g.lines * g.dtaz / 2. + (t - g.tstart) + daz * g.dtaz;
   double etadiff = eta - etaref;

@@ -408,29 +408,40 @@ int s1ifg_geom_create(s1ifg_ctx *ctx, const s1ifg_orbit *orbit, const s1ifg_burs
     recs[(size_t)(k - 1) * 32] = orbit->t[k];
     for (int m = 0; m < 24; ++m) recs[(size_t)(k - 1) * 32 + 1 + m] = coef[(size_t)(k - 1) * 24 + m];
   }
-  size_t nbuf = nrec + 2 * (size_t)N;
+  gd.inv_dmrg = 1.0 / gd.dmrg;
+  gd.inv_dtaz = b->prf;
+  gd.inv_samples = 1.0 / N;
+  gd.inv_lines = 1.0 / L;
+  // buffer: orbit records | kt | etaref | GeomDev copy (256-byte aligned)
+  const size_t ndev = (sizeof(GeomDev) + sizeof(double) - 1) / sizeof(double);
+  const size_t off_dev = ((nrec + 2 * (size_t)N) + 31) & ~(size_t)31;
+  size_t nbuf = off_dev + ndev;
   double *d = nullptr;
   S1_HIP(hipMalloc(&d, nbuf * sizeof(double)));
+  gd.orb.coef = d;
+  gd.orb.t0 = orbit->t[0];
+  gd.orb.h = h;
+  gd.orb.inv_h = 1.0 / h;
+  gd.orb.n = n;
+  gd.kt = d + nrec;
+  gd.etaref = d + nrec + N;
   hipError_t e = hipMemcpy(d, recs.data(), nrec * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(d + nrec, kt.data(), N * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(d + nrec + N, etaref.data(), N * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(d + off_dev, &gd, sizeof(GeomDev), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    hipFree(d);
+    (void)hipFree(d);
     return hip_error(e, "geometry upload");
   }
-  gd.orb.coef = d;
-  gd.orb.t0 = orbit->t[0];
-  gd.orb.h = h;
-  gd.orb.n = n;
-  gd.kt = d + nrec;
-  gd.etaref = d + nrec + N;
 
   s1ifg_geom *g = new s1ifg_geom();
   g->device = ctx->device;
   g->dev = gd;
   g->d_buf = d;
+  g->d_dev = reinterpret_cast<const GeomDev *>(d + off_dev);
   g->n_orb = n;
   g->h_t = new double[n];
   g->h_pos = new double[3 * n];

@@ -86,9 +86,10 @@ __global__ void k_deramp(GeomDev g, const float4 *in2, float4 *out2, const float
   }
 }
 
-__global__ void k_geo2rdr(GeomDev g, OffPoly op, GridDev gr, DemDev dem, const RowTab *rows,
+__global__ void k_geo2rdr(const GeomDev *__restrict__ gp, OffPoly op, GridDev gr, DemDev dem, const RowTab *rows,
                           const ColTab *cols, const float *dh, double *rgoff, double *azoff,
                           double *rng) {
+  const GeomDev &g = *gp;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   int r = blockIdx.y * blockDim.y + threadIdx.y;
   if (r >= gr.nrow || c >= gr.ncol) return;
@@ -97,10 +98,10 @@ __global__ void k_geo2rdr(GeomDev g, OffPoly op, GridDev gr, DemDev dem, const R
   double t, dr[3];
   zero_doppler(g, X, t, dr);
   double R = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-  double rg = (R - g.rngstart) / g.dmrg;
-  double az = (t - g.tstart) / g.dtaz;
+  double rg = (R - g.rngstart) * g.inv_dmrg;
+  double az = (t - g.tstart) * g.inv_dtaz;
   if (op.on) {
-    double xs = rg / g.samples, ys = az / g.lines;
+    double xs = rg * g.inv_samples, ys = az * g.inv_lines;
     double drg = op.p[0] + op.p[1] * xs + op.p[2] * ys;
     double daz = op.p[3] + op.p[4] * xs + op.p[5] * ys;
     rg += drg;
@@ -113,9 +114,10 @@ __global__ void k_geo2rdr(GeomDev g, OffPoly op, GridDev gr, DemDev dem, const R
 }
 
 template <int INTERP>
-__global__ void k_geocode(GeomDev g, OffPoly op, GridDev gr, DemDev dem, const RowTab *rows,
+__global__ void k_geocode(const GeomDev *__restrict__ gp, OffPoly op, GridDev gr, DemDev dem, const RowTab *rows,
                           const ColTab *cols, const float *dh, const float2 *__restrict__ z,
                           float2 *__restrict__ out) {
+  const GeomDev &g = *gp;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   int r = blockIdx.y * blockDim.y + threadIdx.y;
   if (r >= gr.nrow || c >= gr.ncol) return;
@@ -128,11 +130,13 @@ __global__ void k_geocode(GeomDev g, OffPoly op, GridDev gr, DemDev dem, const R
 // accumulated as column means then a row mean (the order of
 // cpx_col_look / cpx_row_look, sario.cu:241-286).
 template <int INTERP>
-__global__ void k_pair(GeomDev gr_, GeomDev gs, OffPoly ops, GridDev gr, DemDev dem,
+__global__ void k_pair(const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr, DemDev dem,
                        const RowTab *rows, const ColTab *cols, const float *dh,
                        const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl,
                        int cl, int nr_ml, int nc_ml, float2 *__restrict__ ifg,
                        float *__restrict__ pr, float *__restrict__ ps) {
+  const GeomDev &gr_ = *gpr;
+  const GeomDev &gs = *gps;
   int cm = blockIdx.x * blockDim.x + threadIdx.x;
   int rm = blockIdx.y * blockDim.y + threadIdx.y;
   if (rm >= nr_ml || cm >= nc_ml) return;
@@ -270,7 +274,7 @@ int s1ifg_geo2rdr(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_grid *grid,
   ColTab *cols;
   if ((rc = grid_tables(ctx, gr, dd, &rows, &cols))) return rc;
   dim3 blk(64, 4), grd((gr.ncol + 63) / 64, (gr.nrow + 3) / 4);
-  hipLaunchKernelGGL(k_geo2rdr, grd, blk, 0, ctx->stream, g->dev, op, gr, dd, rows, cols, dh,
+  hipLaunchKernelGGL(k_geo2rdr, grd, blk, 0, ctx->stream, g->d_dev, op, gr, dd, rows, cols, dh,
                      rgoff, azoff, rng);
   S1_LAUNCH_CHECK("k_geo2rdr");
   return S1IFG_OK;
@@ -292,10 +296,10 @@ int s1ifg_geocode(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_grid *grid,
   if ((rc = grid_tables(ctx, gr, dd, &rows, &cols))) return rc;
   dim3 blk(64, 4), grd((gr.ncol + 63) / 64, (gr.nrow + 3) / 4);
   if (interp == S1IFG_INTERP_BILINEAR)
-    hipLaunchKernelGGL(k_geocode<S1IFG_INTERP_BILINEAR>, grd, blk, 0, ctx->stream, g->dev, op, gr,
+    hipLaunchKernelGGL(k_geocode<S1IFG_INTERP_BILINEAR>, grd, blk, 0, ctx->stream, g->d_dev, op, gr,
                        dd, rows, cols, dh, (const float2 *)deramped, (float2 *)out);
   else
-    hipLaunchKernelGGL(k_geocode<S1IFG_INTERP_SINC>, grd, blk, 0, ctx->stream, g->dev, op, gr, dd,
+    hipLaunchKernelGGL(k_geocode<S1IFG_INTERP_SINC>, grd, blk, 0, ctx->stream, g->d_dev, op, gr, dd,
                        rows, cols, dh, (const float2 *)deramped, (float2 *)out);
   S1_LAUNCH_CHECK("k_geocode");
   return S1IFG_OK;
@@ -337,13 +341,13 @@ int s1ifg_pair(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
   if ((rc = grid_tables(ctx, gr, dd, &rows, &cols))) return rc;
   dim3 blk(64, 4), grd((nc + 63) / 64, (nr + 3) / 4);
   if (interp == S1IFG_INTERP_BILINEAR)
-    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, ctx->stream, g_ref->dev,
-                       g_sec->dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
+    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, ctx->stream, g_ref->d_dev,
+                       g_sec->d_dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
                        (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
                        pow_sec);
   else
-    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_SINC>, grd, blk, 0, ctx->stream, g_ref->dev,
-                       g_sec->dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
+    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_SINC>, grd, blk, 0, ctx->stream, g_ref->d_dev,
+                       g_sec->d_dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
                        (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
                        pow_sec);
   S1_LAUNCH_CHECK("k_pair");

@@ -115,6 +115,63 @@ __global__ void k_coherence(const float2 *__restrict__ ifg, const float *__restr
   coh[o] = g;
 }
 
+// Windowed coherence with the 64 x 4 output tile and its halo staged in LDS
+// (each input element is read from HBM about once); same sums, same order
+// (rows outer, columns inner) as k_coherence.
+constexpr int kCohTX = 64, kCohTY = 4, kCohMaxHalo = 7;
+__global__ __launch_bounds__(256) void k_coherence_tiled(
+    const float2 *__restrict__ ifg, const float *__restrict__ p1, const float *__restrict__ p2,
+    int nr, int nc, int win, int fill, uint32_t seed, float *__restrict__ coh) {
+  constexpr int W = kCohTX + 2 * kCohMaxHalo, H = kCohTY + 2 * kCohMaxHalo;
+  __shared__ float sx[H][W], sy[H][W], s1[H][W], s2[H][W];
+  const int h = win / 2;
+  const int tw = kCohTX + 2 * h, th = kCohTY + 2 * h;
+  const int c0 = blockIdx.x * kCohTX - h, r0 = blockIdx.y * kCohTY - h;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int i = tid; i < tw * th; i += blockDim.x * blockDim.y) {
+    int ty = i / tw, tx = i - ty * tw;
+    int r = r0 + ty, c = c0 + tx;
+    float a = 0.f, b = 0.f, u = 0.f, v = 0.f;
+    if (r >= 0 && r < nr && c >= 0 && c < nc) {
+      size_t o = (size_t)r * nc + c;
+      float2 z = ifg[o];
+      a = z.x;
+      b = z.y;
+      u = p1[o];
+      v = p2[o];
+    }
+    sx[ty][tx] = a;
+    sy[ty][tx] = b;
+    s1[ty][tx] = u;
+    s2[ty][tx] = v;
+  }
+  __syncthreads();
+  const int c = blockIdx.x * kCohTX + threadIdx.x;
+  const int r = blockIdx.y * kCohTY + threadIdx.y;
+  if (r >= nr || c >= nc) return;
+  float ax = 0.f, ay = 0.f, a1 = 0.f, a2 = 0.f;
+  for (int i = 0; i < win; ++i)
+    for (int j = 0; j < win; ++j) {
+      int ty = threadIdx.y + i, tx = threadIdx.x + j;
+      ax += sx[ty][tx];
+      ay += sy[ty][tx];
+      a1 += s1[ty][tx];
+      a2 += s2[ty][tx];
+    }
+  float den = sqrtf(a1 * a2);
+  float g = den > 0.f ? sqrtf(ax * ax + ay * ay) / den : 0.f;
+  size_t o = (size_t)r * nc + c;
+  if (g > 1.0f) {
+    if (fill == S1IFG_COH_FILL_NAN)
+      g = __builtin_nanf("");
+    else if (fill == S1IFG_COH_FILL_HASH)
+      g = hash_uniform(seed, (uint32_t)o);
+    else
+      g = 1.0f;
+  }
+  coh[o] = g;
+}
+
 }  // namespace
 
 extern "C" {
@@ -163,6 +220,12 @@ int s1ifg_coherence(s1ifg_ctx *ctx, const s1ifg_cf32 *ifg_ml, const float *pow_r
   if (fill < S1IFG_COH_FILL_CLAMP || fill > S1IFG_COH_FILL_HASH)
     return set_error(S1IFG_EINVAL, "fill rule %d", fill);
   dim3 blk(64, 4), grd((ncol + 63) / 64, (nrow + 3) / 4);
+  if (win > 1 && win / 2 <= kCohMaxHalo) {
+    hipLaunchKernelGGL(k_coherence_tiled, grd, blk, 0, ctx->stream, (const float2 *)ifg_ml,
+                       pow_ref, pow_sec, nrow, ncol, win, fill, seed, coh);
+    S1_LAUNCH_CHECK("k_coherence_tiled");
+    return S1IFG_OK;
+  }
   hipLaunchKernelGGL(k_coherence, grd, blk, 0, ctx->stream, (const float2 *)ifg_ml, pow_ref,
                      pow_sec, nrow, ncol, win, fill, seed, coh);
   S1_LAUNCH_CHECK("k_coherence");

@@ -37,9 +37,10 @@ int hip_error(hipError_t e, const char *what);
 // [(k-1)*24 + comp*8 + m].
 struct OrbitDev {
   const double *coef;
-  double t0;   // t_0
-  double h;    // t_1 - t_0
-  int n;       // state vectors
+  double t0;     // t_0
+  double h;      // t_1 - t_0
+  double inv_h;  // 1 / h
+  int n;         // state vectors
 };
 
 // Everything a kernel needs about one burst, passed by value.
@@ -50,6 +51,7 @@ struct GeomDev {
   double dc[4];
   double tmid, xmid[3], vmid[3], amid[3];
   double ati;    // azimuth_time_interval (deramp eta)
+  double inv_dmrg, inv_dtaz, inv_samples, inv_lines;
   int lines, samples;
   int flv, llv, fvs, lvs;
   const double *kt;      // [samples]
@@ -85,7 +87,8 @@ struct s1ifg_ctx {
 struct s1ifg_geom {
   int device;
   s1ifg::GeomDev dev;
-  double *d_buf;       // device: orbit coef | kt | etaref
+  double *d_buf;       // device: orbit coef | kt | etaref | GeomDev copy
+  const s1ifg::GeomDev *d_dev;  // device copy of `dev` (for kernels taking pointers)
   // host copies for bounds()
   int n_orb;
   double *h_t, *h_pos, *h_vel;
