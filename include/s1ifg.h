@@ -192,6 +192,16 @@ int s1ifg_crossmul_looks(s1ifg_ctx *ctx, const s1ifg_cf32 *ref, int64_t ld_ref,
                          int ncol, int rl, int cl, s1ifg_cf32 *ifg_ml,
                          float *pow_ref, float *pow_sec);
 
+/* A13 complex look means of one image (column mean over cl, then row mean
+ * over rl) into a (nrow/rl) x (ncol/cl) output.  Replaces cpx_col_look /
+ * cpx_row_look (sario.cu:241-286) and cpxlooks (sario.py:397-413). */
+int s1ifg_looks(s1ifg_ctx *ctx, const s1ifg_cf32 *in, int64_t ld, int nrow, int ncol,
+                int rl, int cl, s1ifg_cf32 *out);
+
+/* A13 phase of n complex values: out = atan2(im, re).  Replaces point_angle
+ * (sario.cu:231-239) used by the daemon's --out-float mode. */
+int s1ifg_phase(s1ifg_ctx *ctx, const s1ifg_cf32 *in, int64_t n, float *out);
+
 /* A16 amplitude looks of one image: amp = sqrt(mean over rl x cl of |s|^2);
  * at rl = cl = 1 the reference returns sqrt(|s|) (sario.py:416-418) and so
  * does this call.  Replaces multilook/powlooks (coherence.py:14-55,

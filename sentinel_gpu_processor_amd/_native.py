@@ -96,6 +96,8 @@ SIGNATURES = {
     "s1ifg_geocode": [_P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P, _I, _P],
     "s1ifg_valid_rows": [_P, _P, _I, _I, _P],
     "s1ifg_crossmul_looks": [_P, _P, _I64, _P, _I64, _I, _I, _I, _I, _P, _P, _P],
+    "s1ifg_looks": [_P, _P, _I64, _I, _I, _I, _I, _P],
+    "s1ifg_phase": [_P, _P, _I64, _P],
     "s1ifg_amplitude": [_P, _P, _I64, _I, _I, _I, _I, _P],
     "s1ifg_coherence": [_P, _P, _P, _P, _I, _I, _I, _I, _U32, _P],
     "s1ifg_goldstein": [_P, _P, _I, _I, _I, _F, _I, _P],

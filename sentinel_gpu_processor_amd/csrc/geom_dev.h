@@ -244,4 +244,82 @@ __device__ __forceinline__ void pixel_xyz(const GridDev &gr, const DemDev &dem,
   target_xyz(rt, ct, h, X);
 }
 
+// deramp_burst element (geo2rdr.cu:138-158): z * exp(i phi),
+// phi = -pi kt (eta - etaref)^2 in FP64, reduced before a float sincos.
+__device__ __forceinline__ float2 deramp_elem(double eta, double kt, double etaref, float2 a) {
+  double d = eta - etaref;
+  float p = reduce_phase(-kPi * kt * d * d);
+  float s, c;
+  sincosf(p, &s, &c);
+  return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+// Grid tables entry i (llh2xyz trigonometry, orbit.cu:15-35, and DEM
+// sampling coordinates).
+__device__ __forceinline__ void grid_table_entry(const GridDev &gr, const DemDev &dem, int i,
+                                                 RowTab *rows, ColTab *cols) {
+  if (i < gr.nrow) {
+    double lat = gr.lat0 + i * gr.dlat;
+    double sl, cl;
+    sincos(lat * kPi / 180, &sl, &cl);
+    RowTab r;
+    r.coslat = cl;
+    r.sinlat = sl;
+    r.re = kRa / sqrt(1. - kRe2 * sl * sl);
+    r.fy = (lat - dem.lat0) / dem.dlat;
+    rows[i] = r;
+  }
+  if (i < gr.ncol) {
+    double lon = gr.lon0 + i * gr.dlon;
+    double sn, cs;
+    sincos(lon * kPi / 180, &sn, &cs);
+    ColTab c;
+    c.coslon = cs;
+    c.sinlon = sn;
+    c.fx = (lon - dem.lon0) / dem.dlon;
+    cols[i] = c;
+  }
+}
+
+// One multilooked pixel of the fused burst pair: geocode both bursts at
+// every target pixel of the rl x cl cell, conj(ref)*sec, |ref|^2, |sec|^2,
+// accumulated as column means then a row mean (the order of cpx_col_look /
+// cpx_row_look, sario.cu:241-286).
+template <int INTERP>
+__device__ __forceinline__ void pair_ml_pixel(const GeomDev &gr_, const GeomDev &gs,
+                                              const OffPoly &ops, const GridDev &gr,
+                                              const DemDev &dem, const RowTab *rows,
+                                              const ColTab *cols, const float *dh,
+                                              const float2 *__restrict__ zr,
+                                              const float2 *__restrict__ zs, int rl, int cl,
+                                              int rm, int cm, int nc_ml, float2 *ifg, float *pr,
+                                              float *ps) {
+  const OffPoly none = {{0, 0, 0, 0, 0, 0}, 0};
+  float ax = 0.f, ay = 0.f, apr = 0.f, aps = 0.f;
+  const float inv_cl = 1.0f / cl, inv_rl = 1.0f / rl;
+  for (int i = 0; i < rl; ++i) {
+    int r = rm * rl + i;
+    float sx = 0.f, sy = 0.f, spr = 0.f, sps = 0.f;
+    for (int j = 0; j < cl; ++j) {
+      int c = cm * cl + j;
+      double X[3];
+      pixel_xyz(gr, dem, rows, cols, dh, r, c, X);
+      float2 a = geocode_pixel<INTERP>(gr_, none, X, zr);
+      float2 b = geocode_pixel<INTERP>(gs, ops, X, zs);
+      sx += a.x * b.x + a.y * b.y;
+      sy += a.x * b.y - a.y * b.x;
+      spr += a.x * a.x + a.y * a.y;
+      sps += b.x * b.x + b.y * b.y;
+    }
+    ax += sx * inv_cl;
+    ay += sy * inv_cl;
+    apr += spr * inv_cl;
+    aps += sps * inv_cl;
+  }
+  size_t o = (size_t)rm * nc_ml + cm;
+  ifg[o] = make_float2(ax * inv_rl, ay * inv_rl);
+  if (pr) pr[o] = apr * inv_rl;
+  if (ps) ps[o] = aps * inv_rl;
+}
+
 }  // namespace s1ifg

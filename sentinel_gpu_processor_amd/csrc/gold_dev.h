@@ -117,4 +117,92 @@ __device__ __forceinline__ void weight_spectrum(float2 v[32], const float S[32],
   }
 }
 
+constexpr int kGoldN = 32;
+constexpr int kGoldPad = 33;
+
+// FP32 patch filter for one 32-lane group (row/column radix-2 FFTs in
+// registers, transposes through the group's LDS planes T and M).  Every
+// thread of the block must call it (it contains block barriers); `active`
+// masks groups without a patch.
+__device__ __forceinline__ void fp32_patch_group(const float2 *__restrict__ img, int nrow,
+                                                 int ncol, int pj_n, long patch, bool active,
+                                                 float alpha, float2 *__restrict__ dst,
+                                                 float2 *Ts, float *Ms, int lane) {
+  const int pi = active ? (int)(patch / pj_n) : 0;
+  const int pj = active ? (int)(patch % pj_n) : 0;
+  const int oy = active ? min(pi * (kGoldN / 4), nrow - kGoldN) : 0;  // goldstein.cu:308-313
+  const int ox = active ? min(pj * (kGoldN / 4), ncol - kGoldN) : 0;
+  float2 v[32];
+  const float2 *src = img + (size_t)(oy + lane) * ncol + ox;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) v[k] = active ? src[k] : make_float2(0.f, 0.f);
+  fft32<false>(v);
+#pragma unroll
+  for (int k = 0; k < 32; ++k) Ts[lane * kGoldPad + k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; ++k) v[k] = Ts[k * kGoldPad + lane];
+  fft32<false>(v);  // v[ky] = Z[ky][lane]
+#pragma unroll
+  for (int k = 0; k < 32; ++k) Ms[k * kGoldPad + lane] = sqrtf(v[k].x * v[k].x + v[k].y * v[k].y);
+  __syncthreads();
+  float S[32];
+  smooth_column(Ms, kGoldPad, lane, S);
+  float sum = 0.f;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) sum += S[k];
+#pragma unroll
+  for (int off = 16; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 32);
+  const float mean = sum / (kGoldN * kGoldN);
+  weight_spectrum(v, S, mean, alpha);
+  fft32<true>(v);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; ++k) Ts[k * kGoldPad + lane] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; ++k) v[k] = Ts[lane * kGoldPad + k];
+  fft32<true>(v);  // v[x] = n^2 * filtered value at (row lane, col x)
+  if (!active) return;
+  float2 *d = dst + lane * kGoldN;
+  const float wy = (float)(lane < kGoldN / 2 ? lane : kGoldN - 1 - lane);
+  const float norm = 1.0f / (kGoldN * kGoldN);
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    float wx = (float)(k < kGoldN / 2 ? k : kGoldN - 1 - k);
+    float w = (wx + wy) * norm;
+    d[k] = make_float2(v[k].x * w, v[k].y * w);
+  }
+}
+
+// Output pixel (y, x): sum of the weighted patches covering it in a fixed
+// order (patch row, then patch column) over the weight sum
+// (goldstein.cu:257-292); 0 where the weight sum is 0.
+__device__ __forceinline__ float2 gold_gather_pixel(const float2 *__restrict__ patches, int nrow,
+                                                    int ncol, int pi_n, int pj_n, int y, int x) {
+  const int step = kGoldN / 4;
+  int pi_lo = y - (kGoldN - 1) > 0 ? (y - (kGoldN - 1) + step - 1) / step : 0;
+  int pj_lo = x - (kGoldN - 1) > 0 ? (x - (kGoldN - 1) + step - 1) / step : 0;
+  float sx = 0.f, sy = 0.f, ws = 0.f;
+  for (int pi = pi_lo; pi < pi_n; ++pi) {
+    int oy = min(pi * step, nrow - kGoldN);
+    if (oy > y) break;
+    if (y >= oy + kGoldN) continue;
+    int ty = y - oy;
+    float wy = (float)(ty < kGoldN / 2 ? ty : kGoldN - 1 - ty);
+    for (int pj = pj_lo; pj < pj_n; ++pj) {
+      int ox = min(pj * step, ncol - kGoldN);
+      if (ox > x) break;
+      if (x >= ox + kGoldN) continue;
+      int tx = x - ox;
+      float wx = (float)(tx < kGoldN / 2 ? tx : kGoldN - 1 - tx);
+      float2 v = patches[((size_t)pi * pj_n + pj) * (kGoldN * kGoldN) + ty * kGoldN + tx];
+      sx += v.x;
+      sy += v.y;
+      ws += wx + wy;
+    }
+  }
+  return ws > 0.f ? make_float2(sx / ws, sy / ws) : make_float2(0.f, 0.f);
+}
+
 }  // namespace s1ifg

@@ -285,10 +285,11 @@ int s1ifg_ctx_create(int device, s1ifg_ctx **out) {
 
 void s1ifg_ctx_destroy(s1ifg_ctx *ctx) {
   if (!ctx) return;
-  hipSetDevice(ctx->device);
+  // Destructors report nothing: errors here leave only a leak.
+  (void)hipSetDevice(ctx->device);
   if (ctx->ws) {
-    hipStreamSynchronize(ctx->stream);
-    hipFree(ctx->ws);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->ws);
   }
   delete ctx;
 }
@@ -455,8 +456,8 @@ int s1ifg_geom_create(s1ifg_ctx *ctx, const s1ifg_orbit *orbit, const s1ifg_burs
 
 void s1ifg_geom_destroy(s1ifg_geom *g) {
   if (!g) return;
-  hipSetDevice(g->device);
-  hipFree(g->d_buf);
+  (void)hipSetDevice(g->device);
+  (void)hipFree(g->d_buf);
   delete[] g->h_t;
   delete[] g->h_pos;
   delete[] g->h_vel;

@@ -1,0 +1,93 @@
+// Device code of the coherence estimators shared by the per-image and
+// batched kernels.
+#pragma once
+
+#include "s1ifg_internal.h"
+
+namespace s1ifg {
+
+__device__ __forceinline__ float hash_uniform(uint32_t seed, uint32_t i) {
+  uint32_t x = i * 0x9E3779B9u ^ seed;
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return (x >> 8) * (1.0f / 16777216.0f);
+}
+
+// Values > 1 (coherence.py:143-144 fills them with unseeded random numbers).
+__device__ __forceinline__ float coh_fill(float g, int fill, uint32_t seed, size_t o) {
+  if (g > 1.0f) {
+    if (fill == S1IFG_COH_FILL_NAN) return __builtin_nanf("");
+    if (fill == S1IFG_COH_FILL_HASH) return hash_uniform(seed, (uint32_t)o);
+    return 1.0f;
+  }
+  return g;
+}
+
+constexpr int kCohTX = 64, kCohTY = 4, kCohMaxHalo = 7;
+constexpr int kCohW = kCohTX + 2 * kCohMaxHalo, kCohH = kCohTY + 2 * kCohMaxHalo;
+
+struct CohTile {
+  float sx[kCohH][kCohW], sy[kCohH][kCohW], s1[kCohH][kCohW], s2[kCohH][kCohW];
+};
+
+// Windowed coherence of one 64 x 4 output tile (block 64 x 4): the tile and
+// its halo are staged in LDS; sums over the window rows outer, columns
+// inner, zero outside the image.  Contains block barriers.
+__device__ __forceinline__ void coherence_tile(const float2 *__restrict__ ifg,
+                                               const float *__restrict__ p1,
+                                               const float *__restrict__ p2, int nr, int nc,
+                                               int win, int fill, uint32_t seed,
+                                               float *__restrict__ coh, int tile_x, int tile_y,
+                                               CohTile &L) {
+  const int h = win / 2;
+  const int tw = kCohTX + 2 * h, th = kCohTY + 2 * h;
+  const int c0 = tile_x * kCohTX - h, r0 = tile_y * kCohTY - h;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int i = tid; i < tw * th; i += blockDim.x * blockDim.y) {
+    int ty = i / tw, tx = i - ty * tw;
+    int r = r0 + ty, c = c0 + tx;
+    float a = 0.f, b = 0.f, u = 0.f, v = 0.f;
+    if (r >= 0 && r < nr && c >= 0 && c < nc) {
+      size_t o = (size_t)r * nc + c;
+      float2 z = ifg[o];
+      a = z.x;
+      b = z.y;
+      u = p1[o];
+      v = p2[o];
+    }
+    L.sx[ty][tx] = a;
+    L.sy[ty][tx] = b;
+    L.s1[ty][tx] = u;
+    L.s2[ty][tx] = v;
+  }
+  __syncthreads();
+  const int c = tile_x * kCohTX + threadIdx.x;
+  const int r = tile_y * kCohTY + threadIdx.y;
+  if (r < nr && c < nc) {
+    float ax = 0.f, ay = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int i = 0; i < win; ++i)
+      for (int j = 0; j < win; ++j) {
+        int ty = threadIdx.y + i, tx = threadIdx.x + j;
+        ax += L.sx[ty][tx];
+        ay += L.sy[ty][tx];
+        a1 += L.s1[ty][tx];
+        a2 += L.s2[ty][tx];
+      }
+    float den = sqrtf(a1 * a2);
+    float g = den > 0.f ? sqrtf(ax * ax + ay * ay) / den : 0.f;
+    size_t o = (size_t)r * nc + c;
+    coh[o] = coh_fill(g, fill, seed, o);
+  }
+  __syncthreads();
+}
+
+// Look-cell estimator (coherence.py:142): |ifg| / (sqrt(p1)+eps) / (sqrt(p2)+eps).
+__device__ __forceinline__ float coherence_cell(float2 v, float p1, float p2) {
+  float a1 = sqrtf(p1) + 1e-12f, a2 = sqrtf(p2) + 1e-12f;
+  return sqrtf(v.x * v.x + v.y * v.y) / a1 / a2;
+}
+
+}  // namespace s1ifg

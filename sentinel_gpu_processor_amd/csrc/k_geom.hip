@@ -10,28 +10,7 @@ namespace {
 // (orbit.cu:15-35) and the DEM sampling coordinates.
 __global__ void k_grid_tables(GridDev gr, DemDev dem, RowTab *rows, ColTab *cols) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < gr.nrow) {
-    double lat = gr.lat0 + i * gr.dlat;
-    double rl = lat * kPi / 180;
-    double sl, cl;
-    sincos(rl, &sl, &cl);
-    RowTab r;
-    r.coslat = cl;
-    r.sinlat = sl;
-    r.re = kRa / sqrt(1. - kRe2 * sl * sl);
-    r.fy = (lat - dem.lat0) / dem.dlat;
-    rows[i] = r;
-  }
-  if (i < gr.ncol) {
-    double lon = gr.lon0 + i * gr.dlon;
-    double sn, cs;
-    sincos(lon * kPi / 180, &sn, &cs);
-    ColTab c;
-    c.coslon = cs;
-    c.sinlon = sn;
-    c.fx = (lon - dem.lon0) / dem.dlon;
-    cols[i] = c;
-  }
+  grid_table_entry(gr, dem, i, rows, cols);
 }
 
 __global__ void k_orbit_eval(OrbitDev o, const double *t, long n, double *pos, double *vel,
@@ -61,26 +40,12 @@ __global__ void k_deramp(GeomDev g, const float4 *in2, float4 *out2, const float
     int j = 2 * jj;
     if (vec) {
       float4 a = in2[(rowoff >> 1) + jj];
-      double d0 = eta - g.etaref[j], d1 = eta - g.etaref[j + 1];
-      float p0 = reduce_phase(-kPi * g.kt[j] * d0 * d0);
-      float p1 = reduce_phase(-kPi * g.kt[j + 1] * d1 * d1);
-      float s0, c0, s1, c1;
-      sincosf(p0, &s0, &c0);
-      sincosf(p1, &s1, &c1);
-      float4 o;
-      o.x = a.x * c0 - a.y * s0;
-      o.y = a.x * s0 + a.y * c0;
-      o.z = a.z * c1 - a.w * s1;
-      o.w = a.z * s1 + a.w * c1;
-      out2[(rowoff >> 1) + jj] = o;
+      float2 o0 = deramp_elem(eta, g.kt[j], g.etaref[j], make_float2(a.x, a.y));
+      float2 o1 = deramp_elem(eta, g.kt[j + 1], g.etaref[j + 1], make_float2(a.z, a.w));
+      out2[(rowoff >> 1) + jj] = make_float4(o0.x, o0.y, o1.x, o1.y);
     } else {
       for (int k = j; k < j + 2 && k < N; ++k) {
-        float2 a = in1[rowoff + k];
-        double d = eta - g.etaref[k];
-        float p = reduce_phase(-kPi * g.kt[k] * d * d);
-        float s, c;
-        sincosf(p, &s, &c);
-        out1[rowoff + k] = make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+        out1[rowoff + k] = deramp_elem(eta, g.kt[k], g.etaref[k], in1[rowoff + k]);
       }
     }
   }
@@ -135,37 +100,11 @@ __global__ void k_pair(const GeomDev *__restrict__ gpr, const GeomDev *__restric
                        const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl,
                        int cl, int nr_ml, int nc_ml, float2 *__restrict__ ifg,
                        float *__restrict__ pr, float *__restrict__ ps) {
-  const GeomDev &gr_ = *gpr;
-  const GeomDev &gs = *gps;
   int cm = blockIdx.x * blockDim.x + threadIdx.x;
   int rm = blockIdx.y * blockDim.y + threadIdx.y;
   if (rm >= nr_ml || cm >= nc_ml) return;
-  const OffPoly none = {{0, 0, 0, 0, 0, 0}, 0};
-  float ax = 0.f, ay = 0.f, apr = 0.f, aps = 0.f;
-  const float inv_cl = 1.0f / cl, inv_rl = 1.0f / rl;
-  for (int i = 0; i < rl; ++i) {
-    int r = rm * rl + i;
-    float sx = 0.f, sy = 0.f, spr = 0.f, sps = 0.f;
-    for (int j = 0; j < cl; ++j) {
-      int c = cm * cl + j;
-      double X[3];
-      pixel_xyz(gr, dem, rows, cols, dh, r, c, X);
-      float2 a = geocode_pixel<INTERP>(gr_, none, X, zr);
-      float2 b = geocode_pixel<INTERP>(gs, ops, X, zs);
-      sx += a.x * b.x + a.y * b.y;
-      sy += a.x * b.y - a.y * b.x;
-      spr += a.x * a.x + a.y * a.y;
-      sps += b.x * b.x + b.y * b.y;
-    }
-    ax += sx * inv_cl;
-    ay += sy * inv_cl;
-    apr += spr * inv_cl;
-    aps += sps * inv_cl;
-  }
-  size_t o = (size_t)rm * nc_ml + cm;
-  ifg[o] = make_float2(ax * inv_rl, ay * inv_rl);
-  if (pr) pr[o] = apr * inv_rl;
-  if (ps) ps[o] = aps * inv_rl;
+  pair_ml_pixel<INTERP>(*gpr, *gps, ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, rm, cm, nc_ml,
+                        ifg, pr, ps);
 }
 
 __global__ void k_rows_init(int *fl, int nrow) {

@@ -1,5 +1,6 @@
 // Pair products on the target / multilooked grid: fused crossmul + looks,
 // amplitude looks, coherence.
+#include "ifg_dev.h"
 #include "s1ifg_internal.h"
 
 using namespace s1ifg;
@@ -40,6 +41,33 @@ __global__ void k_crossmul_looks(const float2 *__restrict__ ref, long ld_ref,
   if (ps) ps[o] = aps * inv_rl;
 }
 
+// cpx_col_look + cpx_row_look (sario.cu:241-286) on one image.
+__global__ void k_looks(const float2 *__restrict__ s, long ld, int rl, int cl, int nr, int nc,
+                        float2 *__restrict__ out) {
+  int cm = blockIdx.x * blockDim.x + threadIdx.x;
+  int rm = blockIdx.y * blockDim.y + threadIdx.y;
+  if (rm >= nr || cm >= nc) return;
+  const float inv_cl = 1.0f / cl, inv_rl = 1.0f / rl;
+  float ax = 0.f, ay = 0.f;
+  for (int i = 0; i < rl; ++i) {
+    const float2 *a = s + (size_t)(rm * rl + i) * ld + (size_t)cm * cl;
+    float sx = 0.f, sy = 0.f;
+    for (int j = 0; j < cl; ++j) {
+      sx += a[j].x;
+      sy += a[j].y;
+    }
+    ax += sx * inv_cl;
+    ay += sy * inv_cl;
+  }
+  out[(size_t)rm * nc + cm] = make_float2(ax * inv_rl, ay * inv_rl);
+}
+
+// point_angle (sario.cu:231-239).
+__global__ void k_phase(const float2 *__restrict__ z, long n, float *__restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = atan2f(z[i].y, z[i].x);
+}
+
 // multilook/powlooks (coherence.py:46-48, sario.py:416-420).
 __global__ void k_amplitude(const float2 *__restrict__ s, long ld, int rl, int cl, int nr,
                             int nc, float *__restrict__ amp) {
@@ -63,16 +91,6 @@ __global__ void k_amplitude(const float2 *__restrict__ s, long ld, int rl, int c
   amp[o] = sqrtf(acc * inv_rl);
 }
 
-__device__ __forceinline__ float hash_uniform(uint32_t seed, uint32_t i) {
-  uint32_t x = i * 0x9E3779B9u ^ seed;
-  x ^= x >> 16;
-  x *= 0x7feb352du;
-  x ^= x >> 15;
-  x *= 0x846ca68bu;
-  x ^= x >> 16;
-  return (x >> 8) * (1.0f / 16777216.0f);
-}
-
 // coherence.py:142-144 (win = 1) and the windowed estimator (win >= 3).
 __global__ void k_coherence(const float2 *__restrict__ ifg, const float *__restrict__ p1,
                             const float *__restrict__ p2, int nr, int nc, int win, int fill,
@@ -83,9 +101,7 @@ __global__ void k_coherence(const float2 *__restrict__ ifg, const float *__restr
   size_t o = (size_t)r * nc + c;
   float g;
   if (win == 1) {
-    float2 v = ifg[o];
-    float a1 = sqrtf(p1[o]) + 1e-12f, a2 = sqrtf(p2[o]) + 1e-12f;
-    g = sqrtf(v.x * v.x + v.y * v.y) / a1 / a2;
+    g = coherence_cell(ifg[o], p1[o], p2[o]);
   } else {
     int h = win / 2;
     int r0 = max(r - h, 0), r1 = min(r + h, nr - 1);
@@ -104,72 +120,17 @@ __global__ void k_coherence(const float2 *__restrict__ ifg, const float *__restr
     float den = sqrtf(s1 * s2);
     g = den > 0.f ? sqrtf(sx * sx + sy * sy) / den : 0.f;
   }
-  if (g > 1.0f) {
-    if (fill == S1IFG_COH_FILL_NAN)
-      g = __builtin_nanf("");
-    else if (fill == S1IFG_COH_FILL_HASH)
-      g = hash_uniform(seed, (uint32_t)o);
-    else
-      g = 1.0f;
-  }
-  coh[o] = g;
+  coh[o] = coh_fill(g, fill, seed, o);
 }
 
 // Windowed coherence with the 64 x 4 output tile and its halo staged in LDS
 // (each input element is read from HBM about once); same sums, same order
 // (rows outer, columns inner) as k_coherence.
-constexpr int kCohTX = 64, kCohTY = 4, kCohMaxHalo = 7;
 __global__ __launch_bounds__(256) void k_coherence_tiled(
     const float2 *__restrict__ ifg, const float *__restrict__ p1, const float *__restrict__ p2,
     int nr, int nc, int win, int fill, uint32_t seed, float *__restrict__ coh) {
-  constexpr int W = kCohTX + 2 * kCohMaxHalo, H = kCohTY + 2 * kCohMaxHalo;
-  __shared__ float sx[H][W], sy[H][W], s1[H][W], s2[H][W];
-  const int h = win / 2;
-  const int tw = kCohTX + 2 * h, th = kCohTY + 2 * h;
-  const int c0 = blockIdx.x * kCohTX - h, r0 = blockIdx.y * kCohTY - h;
-  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  for (int i = tid; i < tw * th; i += blockDim.x * blockDim.y) {
-    int ty = i / tw, tx = i - ty * tw;
-    int r = r0 + ty, c = c0 + tx;
-    float a = 0.f, b = 0.f, u = 0.f, v = 0.f;
-    if (r >= 0 && r < nr && c >= 0 && c < nc) {
-      size_t o = (size_t)r * nc + c;
-      float2 z = ifg[o];
-      a = z.x;
-      b = z.y;
-      u = p1[o];
-      v = p2[o];
-    }
-    sx[ty][tx] = a;
-    sy[ty][tx] = b;
-    s1[ty][tx] = u;
-    s2[ty][tx] = v;
-  }
-  __syncthreads();
-  const int c = blockIdx.x * kCohTX + threadIdx.x;
-  const int r = blockIdx.y * kCohTY + threadIdx.y;
-  if (r >= nr || c >= nc) return;
-  float ax = 0.f, ay = 0.f, a1 = 0.f, a2 = 0.f;
-  for (int i = 0; i < win; ++i)
-    for (int j = 0; j < win; ++j) {
-      int ty = threadIdx.y + i, tx = threadIdx.x + j;
-      ax += sx[ty][tx];
-      ay += sy[ty][tx];
-      a1 += s1[ty][tx];
-      a2 += s2[ty][tx];
-    }
-  float den = sqrtf(a1 * a2);
-  float g = den > 0.f ? sqrtf(ax * ax + ay * ay) / den : 0.f;
-  size_t o = (size_t)r * nc + c;
-  if (g > 1.0f) {
-    if (fill == S1IFG_COH_FILL_NAN)
-      g = __builtin_nanf("");
-    else if (fill == S1IFG_COH_FILL_HASH)
-      g = hash_uniform(seed, (uint32_t)o);
-    else
-      g = 1.0f;
-  }
-  coh[o] = g;
+  __shared__ CohTile L;
+  coherence_tile(ifg, p1, p2, nr, nc, win, fill, seed, coh, blockIdx.x, blockIdx.y, L);
 }
 
 }  // namespace
@@ -191,6 +152,32 @@ int s1ifg_crossmul_looks(s1ifg_ctx *ctx, const s1ifg_cf32 *ref, int64_t ld_ref,
                      (long)ld_ref, (const float2 *)sec, (long)ld_sec, rl, cl, nr, nc,
                      (float2 *)ifg_ml, pow_ref, pow_sec);
   S1_LAUNCH_CHECK("k_crossmul_looks");
+  return S1IFG_OK;
+}
+
+int s1ifg_looks(s1ifg_ctx *ctx, const s1ifg_cf32 *in, int64_t ld, int nrow, int ncol, int rl,
+                int cl, s1ifg_cf32 *out) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!in || !out) return set_error(S1IFG_EINVAL, "null data pointer");
+  if (rl < 1 || cl < 1) return set_error(S1IFG_EINVAL, "looks %d x %d", rl, cl);
+  if (nrow < rl || ncol < cl || ld < ncol) return set_error(S1IFG_EINVAL, "bad image size / stride");
+  int nr = nrow / rl, nc = ncol / cl;
+  dim3 blk(64, 4), grd((nc + 63) / 64, (nr + 3) / 4);
+  hipLaunchKernelGGL(k_looks, grd, blk, 0, ctx->stream, (const float2 *)in, (long)ld, rl, cl, nr,
+                     nc, (float2 *)out);
+  S1_LAUNCH_CHECK("k_looks");
+  return S1IFG_OK;
+}
+
+int s1ifg_phase(s1ifg_ctx *ctx, const s1ifg_cf32 *in, int64_t n, float *out) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!in || !out))) return set_error(S1IFG_EINVAL, "bad arguments");
+  if (n == 0) return S1IFG_OK;
+  hipLaunchKernelGGL(k_phase, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (const float2 *)in, (long)n, out);
+  S1_LAUNCH_CHECK("k_phase");
   return S1IFG_OK;
 }
 

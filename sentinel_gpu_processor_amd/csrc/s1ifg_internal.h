@@ -102,4 +102,5 @@ int check_geom(s1ifg_ctx *ctx, const s1ifg_geom *g);
 int make_grid(const s1ifg_grid *grid, GridDev *out);
 int make_dem(const s1ifg_dem *dem, DemDev *out);
 int make_offpoly(const double *p, OffPoly *out);
+int device_cu_count(int device);
 }  // namespace s1ifg

@@ -97,6 +97,23 @@ def crossmul_looks(ctx, ref, sec, rl, cl, with_power=True, out=None):
     return ifg, pr, ps
 
 
+def looks(ctx, img, rl, cl):
+    """Complex look means (sario.cu:241-286)."""
+    _need(img, torch.complex64, None, "image")
+    nrow, ncol = img.shape
+    out = _new((nrow // rl, ncol // cl), torch.complex64, img.device)
+    N.check(ctx.lib.s1ifg_looks(ctx.handle, N.ptr(img), ncol, nrow, ncol, rl, cl, N.ptr(out)))
+    return out
+
+
+def phase(ctx, z):
+    """atan2 of a complex image (sario.cu:231-239)."""
+    _need(z, torch.complex64, None, "image")
+    out = _new(tuple(z.shape), torch.float32, z.device)
+    N.check(ctx.lib.s1ifg_phase(ctx.handle, N.ptr(z), z.numel(), N.ptr(out)))
+    return out
+
+
 def amplitude(ctx, slc, rl, cl):
     """sqrt of power looks (coherence.py:46-48; 1x1 quirk sqrt|s|)."""
     _need(slc, torch.complex64, None, "slc")
