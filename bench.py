@@ -6,10 +6,17 @@ reramp + flattening of both, crossmul with 1x4 looks, Goldstein filter
 (alpha 0.5, 32/8) and 5x5 windowed coherence.
 
 Forms:
-  plain  per-stage kernels (deramp x2, geocode x2 to full-resolution
-         geocoded SLCs, crossmul+looks, Goldstein FP32, coherence)
-  fused  deramp x2, fused pair kernel (geocoded SLCs never written),
-         Goldstein (FP16-split MFMA when built, else FP32), coherence
+  plain    per-stage kernels (deramp x2, geocode x2 to full-resolution
+           geocoded SLCs, crossmul+looks, Goldstein FP32, coherence)
+  fused    deramp x2, fused pair kernel (geocoded SLCs never written),
+           Goldstein (FP16-split MFMA when built, else FP32), coherence
+  batched  the fused chain as persistent batch kernels replayed from one
+           HIP graph, launch shape from the autotuner
+
+Besides the one-pair workload, a synthetic IW pair of 27 bursts of uneven
+size (3 subswaths x 9, 1450-1530 x 20000-25500) is timed in the plain form
+(per-burst loop), the fused form (per-burst loop) and the batched form (one
+graph launch for all 27 bursts); it is reported under "iw".
 
 Prints one JSON line (rank 0).  Timing uses HIP events on the stream the
 library launches on.
@@ -36,7 +43,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--config", default="c2", choices=["c1", "c2"])
     p.add_argument("--interp", default="sinc", choices=["bilinear", "sinc"])
-    p.add_argument("--forms", default="plain,fused")
+    p.add_argument("--forms", default="plain,fused,batched")
+    p.add_argument("--no-iw", action="store_true", help="skip the 27-burst IW pair")
+    p.add_argument("--iw-steps", type=int, default=5)
     p.add_argument("--gold-mode", default="auto", choices=["auto", "fp32", "fp16_split"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", default="c1")
@@ -135,32 +144,165 @@ def run_gpu(args):
         K.coherence(ctx, ifg, pr, ps, win=5, out=coh)
         ev.mark(None)
 
-    forms = {"plain": plain, "fused": fused}
+    holder = {}
+
+    def batched(ev):
+        ev.mark("batch")
+        holder["batch"].run(graph=True)
+        ev.mark(None)
+
+    forms = {"plain": plain, "fused": fused, "batched": batched}
+    wanted = [f for f in args.forms.split(",") if f]
     results = {}
-    for name in [f for f in args.forms.split(",") if f]:
-        fn = forms[name]
-        for _ in range(args.warmup):
-            fn(NoEvents())
-        torch.cuda.synchronize()
-        ev = StageEvents()
-        start = torch.cuda.Event(enable_timing=True)
-        stop = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        start.record()
-        for _ in range(args.steps):
-            fn(ev)
-        stop.record()
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        ms = start.elapsed_time(stop) / args.steps
-        results[name] = dict(ms_per_step=ms, wall_s=wall, stages=ev.averages(args.steps))
-        print(f"[bench] {name}: {ms:.3f} ms/step  stages(ms/step) "
-              + " ".join(f"{k}={v:.3f}" for k, v in results[name]["stages"].items()),
+    if "batched" in wanted:
+        from sentinel_gpu_processor_amd import autotune
+        from sentinel_gpu_processor_amd.batch import PairBatch
+        b = PairBatch(ctx, [dict(g_ref=gr, g_sec=gs, grid=grid, dem_t=dem_t, dem=dem,
+                                 ref_slc=ref, sec_slc=sec, offpoly=offpoly, ref_deramped=dref,
+                                 sec_deramped=dsec, ifg=ifg, pow_ref=pr, pow_sec=ps, filt=filt,
+                                 coh=coh)],
+                      interp=args.interp, rl=rl, cl=cl, alpha=0.5, gold_mode=gold_mode,
+                      coh_win=5, coh_fill="clamp")
+        shape, tms, _ = autotune.tune(b, reps=2)
+        print(f"[bench] batched launch shape {shape} ({tms:.3f} ms while tuning)",
               file=sys.stderr, flush=True)
+        holder["batch"] = b
+        results["_launch"] = shape
+    for name in wanted:
+        results[name] = time_form(forms[name], args.steps, args.warmup, name)
     results["_gold_mode"] = gold_mode
     results["_shape"] = (L, Nr, dem["data"].size, rl, cl)
+    if "batch" in holder:
+        holder["batch"].close()
     return results
+
+
+def time_form(fn, steps, warmup, name):
+    import torch
+    for _ in range(warmup):
+        fn(NoEvents())
+    torch.cuda.synchronize()
+    ev = StageEvents()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(steps):
+        fn(ev)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(stop) / steps
+    r = dict(ms_per_step=ms, wall_s=wall, stages=ev.averages(steps))
+    print(f"[bench] {name}: {ms:.3f} ms/step  stages(ms/step) "
+          + " ".join(f"{k}={v:.3f}" for k, v in r["stages"].items()), file=sys.stderr, flush=True)
+    return r
+
+
+def run_iw(args, gold_mode):
+    """27-burst IW pair: plain and fused per-burst loops vs one batched graph."""
+    import torch
+
+    from sentinel_gpu_processor_amd import _native as N
+    from sentinel_gpu_processor_amd import autotune
+    from sentinel_gpu_processor_amd import kernels as K
+    from sentinel_gpu_processor_amd import synthetic as S
+    from sentinel_gpu_processor_amd.batch import PairBatch
+
+    dev = torch.device("cuda:0")
+    cfg = S.CONFIGS["c3"]
+    t0 = time.perf_counter()
+    scenes = S.make_iw(seed=0, n_sub=cfg["n_sub"], n_burst=cfg["n_burst"], lines=cfg["lines"],
+                       samples=cfg["samples"], dem_shape=cfg["dem"], dem_kind=cfg["dem_kind"],
+                       hmax=cfg["hmax"], misreg=cfg["offset"], fidelity="fast", device=dev)
+    print(f"[bench] iw: generated {len(scenes)} burst pairs in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    ctx = N.Context(0)
+    rl, cl = 1, 4
+    pairs = []
+    for sc in scenes:
+        g = sc["grid"]
+        nml = (g["nrow"] // rl, g["ncol"] // cl)
+        pairs.append(dict(
+            g_ref=N.Geometry(ctx, sc["orbit_ref"], sc["burst_ref"]),
+            g_sec=N.Geometry(ctx, sc["orbit_sec"], sc["burst_sec"]),
+            grid=g, dem=sc["dem"], dem_t=torch.from_numpy(sc["dem"]["data"]).to(dev),
+            ref_slc=sc["ref_slc"], sec_slc=sc["sec_slc"], offpoly=sc["offpoly"],
+            ref_deramped=torch.empty_like(sc["ref_slc"]),
+            sec_deramped=torch.empty_like(sc["sec_slc"]),
+            ifg=torch.empty(nml, dtype=torch.complex64, device=dev),
+            pow_ref=torch.empty(nml, dtype=torch.float32, device=dev),
+            pow_sec=torch.empty(nml, dtype=torch.float32, device=dev),
+            filt=torch.empty(nml, dtype=torch.complex64, device=dev),
+            coh=torch.empty(nml, dtype=torch.float32, device=dev)))
+    npx = sum(p["grid"]["nrow"] * p["grid"]["ncol"] for p in pairs)
+    big = max(p["grid"]["nrow"] * p["grid"]["ncol"] for p in pairs)
+    scratch_r = torch.empty(big, dtype=torch.complex64, device=dev)
+    scratch_s = torch.empty(big, dtype=torch.complex64, device=dev)
+
+    def plain(ev):
+        ev.mark("plain")
+        for p in pairs:
+            g = p["grid"]
+            g_r = scratch_r[:g["nrow"] * g["ncol"]].view(g["nrow"], g["ncol"])
+            g_s = scratch_s[:g["nrow"] * g["ncol"]].view(g["nrow"], g["ncol"])
+            K.deramp(ctx, p["g_ref"], p["ref_slc"], out=p["ref_deramped"])
+            K.deramp(ctx, p["g_sec"], p["sec_slc"], out=p["sec_deramped"])
+            K.geocode(ctx, p["g_ref"], g, p["dem_t"], p["dem"], p["ref_deramped"], args.interp,
+                      out=g_r)
+            K.geocode(ctx, p["g_sec"], g, p["dem_t"], p["dem"], p["sec_deramped"], args.interp,
+                      offpoly=p["offpoly"], out=g_s)
+            crossmul_into(K, ctx, g_r, g_s, rl, cl, p["ifg"], p["pow_ref"], p["pow_sec"])
+            K.goldstein(ctx, p["ifg"], 0.5, mode="fp32", out=p["filt"])
+            K.coherence(ctx, p["ifg"], p["pow_ref"], p["pow_sec"], win=5, out=p["coh"])
+        ev.mark(None)
+
+    def fused(ev):
+        ev.mark("fused")
+        for p in pairs:
+            K.deramp(ctx, p["g_ref"], p["ref_slc"], out=p["ref_deramped"])
+            K.deramp(ctx, p["g_sec"], p["sec_slc"], out=p["sec_deramped"])
+            K.pair(ctx, p["g_ref"], p["g_sec"], p["grid"], p["dem_t"], p["dem"],
+                   p["ref_deramped"], p["sec_deramped"], args.interp, rl, cl,
+                   offpoly=p["offpoly"], out=(p["ifg"], p["pow_ref"], p["pow_sec"]))
+            K.goldstein(ctx, p["ifg"], 0.5, mode=gold_mode, out=p["filt"])
+            K.coherence(ctx, p["ifg"], p["pow_ref"], p["pow_sec"], win=5, out=p["coh"])
+        ev.mark(None)
+
+    b = PairBatch(ctx, pairs, interp=args.interp, rl=rl, cl=cl, alpha=0.5, gold_mode=gold_mode,
+                  coh_win=5, coh_fill="clamp")
+    shape, tms, _ = autotune.tune(b, reps=1)
+    print(f"[bench] iw batched launch shape {shape} ({tms:.3f} ms while tuning)",
+          file=sys.stderr, flush=True)
+
+    def batched(ev):
+        ev.mark("batched")
+        b.run(graph=True)
+        ev.mark(None)
+
+    out = {}
+    for name, fn in (("plain", plain), ("fused", fused), ("batched", batched)):
+        r = time_form(fn, args.iw_steps, 1, "iw " + name)
+        out[name] = dict(ms_per_step=round(r["ms_per_step"], 3),
+                         mpix_s=round(npx / (r["ms_per_step"] * 1e-3) / 1e6, 1))
+    dem_px = sum(p["dem"]["data"].size for p in pairs)
+    nml = sum((p["grid"]["nrow"] // rl) * (p["grid"]["ncol"] // cl) for p in pairs)
+    balg = 16 * npx + 4 * dem_px + 20 * nml
+    for v in out.values():
+        v["hbm_frac"] = round(balg / (v["ms_per_step"] * 1e-3) / HBM_PEAK, 4)
+    res = dict(workload=f"c3: {len(pairs)} burst pairs (3 subswaths x 9), "
+                        f"{min(p['grid']['nrow'] for p in pairs)}-"
+                        f"{max(p['grid']['nrow'] for p in pairs)} x "
+                        f"{min(p['grid']['ncol'] for p in pairs)}-"
+                        f"{max(p['grid']['ncol'] for p in pairs)}, {args.interp}, {gold_mode}",
+               target_px=npx, b_alg_bytes=balg, launch=shape, forms=out,
+               batched_vs_plain=round(out["plain"]["ms_per_step"] / out["batched"]["ms_per_step"], 3),
+               batched_vs_fused_loop=round(out["fused"]["ms_per_step"]
+                                           / out["batched"]["ms_per_step"], 3),
+               scratch_bytes=b.scratch_bytes())
+    b.close()
+    return res
 
 
 def crossmul_into(K, ctx, a, b, rl, cl, ifg, pr, ps):
@@ -247,7 +389,7 @@ def main():
     balg = b_alg(L, Nr, dem_px, rl, cl)
     sb = stage_bytes(L, Nr, dem_px, rl, cl)
     forms = {}
-    for name in ("plain", "fused"):
+    for name in ("plain", "fused", "batched"):
         if name not in res:
             continue
         r = res[name]
@@ -255,10 +397,11 @@ def main():
         forms[name] = dict(ms_per_step=round(ms, 4), mpix_s=round(npx / (ms * 1e-3) / 1e6, 1),
                            hbm_frac=round(balg / (ms * 1e-3) / HBM_PEAK, 4),
                            stages_ms={k: round(v, 4) for k, v in r["stages"].items()})
-    best = "fused" if "fused" in forms else "plain"
+    best = min(forms, key=lambda k: forms[k]["ms_per_step"])
     fb = forms[best]
-    r = res[best]
-    # dominant kernel: the stage with the largest share of the step
+    # dominant kernel: the stage with the largest share of the fused step
+    # (the batched form is one graph launch of the same device code)
+    r = res["fused"] if "fused" in res else res[best]
     dom = max(r["stages"], key=r["stages"].get)
     per_launch = {"deramp": 2, "geocode": 2}.get(dom, 1)
     t_launch = r["stages"][dom] / per_launch * 1e-3
@@ -266,6 +409,12 @@ def main():
     roof = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK / 1e9,
                 unit="GB/s", frac=round(achieved * 1e9 / HBM_PEAK, 4), traffic=None,
                 alg_bytes_per_launch=sb[dom])
+    iw = None
+    if not args.no_iw:
+        try:
+            iw = run_iw(args, res["_gold_mode"])
+        except Exception as e:  # reported; the one-pair number stands alone
+            iw = dict(error=repr(e))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -292,6 +441,8 @@ def main():
         "hbm_frac_path": fb["hbm_frac"],
         "b_alg_bytes": balg,
         "forms": forms,
+        "launch": res.get("_launch"),
+        "iw": iw,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

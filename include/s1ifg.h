@@ -240,6 +240,68 @@ int s1ifg_pair(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
                const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl,
                s1ifg_cf32 *ifg_ml, float *pow_ref, float *pow_sec);
 
+/* ------------------------------------------------------------ batches */
+/* One burst pair of a batch (all pointers DEVICE except the geometries).
+ * The raw bursts are deramped into ref_deramped / sec_deramped, which must
+ * not alias the raw bursts (a batch may be run many times).  The products
+ * are on the multilooked grid (grid.nrow/rl x grid.ncol/cl); filt and coh
+ * may be NULL to skip the Goldstein filter / coherence of that pair. */
+typedef struct {
+  const s1ifg_geom *g_ref, *g_sec;
+  s1ifg_grid grid;
+  s1ifg_dem dem;
+  const float *dh;           /* per target pixel height error, or NULL */
+  double offpoly_sec[6];     /* secondary offset polynomial */
+  int32_t use_offpoly;
+  int32_t pad_;
+  const s1ifg_cf32 *ref_slc, *sec_slc;
+  s1ifg_cf32 *ref_deramped, *sec_deramped;
+  s1ifg_cf32 *ifg_ml;
+  float *pow_ref, *pow_sec;
+  s1ifg_cf32 *filt;
+  float *coh;
+} s1ifg_pair_task;
+
+/* Processing parameters shared by the pairs of a batch, and the launch
+ * shape (0 = built-in default; an autotuner measures and picks them). */
+typedef struct {
+  int32_t interp, rl, cl;
+  float alpha;
+  int32_t gold_mode, coh_win, coh_fill;
+  uint32_t coh_seed;
+  int32_t pair_block_x;      /* pair tile width in multilooked pixels: 16..256 */
+  int32_t pair_blocks_per_cu;/* persistent grid of the pair kernel */
+  int32_t gold_blocks_per_cu;/* persistent grid of the Goldstein patch pass */
+  int32_t stream_blocks_per_cu; /* deramp / overlap-add / coherence */
+} s1ifg_batch_opts;
+
+typedef struct s1ifg_batch s1ifg_batch;
+
+/* All bursts of an IW pair (or of several pairs of a stack) in one launch
+ * sequence of persistent kernels: deramp of every burst, the fused pair
+ * kernel, the Goldstein patch pass, its overlap-add, coherence.  Each pair's
+ * products are bit-identical to s1ifg_deramp x2 + s1ifg_pair +
+ * s1ifg_goldstein + s1ifg_coherence on that pair alone.  The batch owns its
+ * scratch memory (tables, descriptors, Goldstein patches).  Replaces the
+ * per-burst loops of geo2rdr (geo2rdr.cu:552-664) and the crossmul daemon's
+ * lanes (crossmul_daemon.cu:921-1117) for the same pairs. */
+int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
+                       const s1ifg_batch_opts *opts, s1ifg_batch **out);
+void s1ifg_batch_destroy(s1ifg_batch *b);
+/* Enqueue the batch on the context stream.  use_graph != 0: the launch
+ * sequence is captured once into a HIP graph (on the batch's own stream)
+ * and replayed as one graph launch afterwards. */
+int s1ifg_batch_run(s1ifg_batch *b, int use_graph);
+/* Device bytes held by the batch. */
+int s1ifg_batch_bytes(const s1ifg_batch *b, int64_t *bytes);
+/* The launch shape in use (after defaults are filled in). */
+int s1ifg_batch_get_opts(const s1ifg_batch *b, s1ifg_batch_opts *opts);
+/* Change the launch shape (the pair_block_x / *_blocks_per_cu fields of
+ * opts; the processing parameters are kept).  Waits for the batch's pending
+ * work; the next graph run captures a new graph.  Results do not depend on
+ * the launch shape. */
+int s1ifg_batch_set_launch(s1ifg_batch *b, const s1ifg_batch_opts *opts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,7 +16,8 @@ LIBDIR = os.path.join(PKG, "lib")
 LIBNAME = "libs1ifg.so"
 ARCH = os.environ.get("S1IFG_ARCH", "gfx950")
 
-SOURCES = ["host.cpp", "k_geom.hip", "k_ifg.hip", "k_goldstein.hip", "k_gold_mfma.hip"]
+SOURCES = ["host.cpp", "k_geom.hip", "k_ifg.hip", "k_goldstein.hip", "k_gold_mfma.hip",
+           "k_batch.hip"]
 
 HIPFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -70,6 +70,30 @@ class Dem(ctypes.Structure):
                 ("dlat", ctypes.c_double), ("dlon", ctypes.c_double)]
 
 
+class PairTask(ctypes.Structure):
+    _fields_ = [("g_ref", ctypes.c_void_p), ("g_sec", ctypes.c_void_p),
+                ("grid", Grid), ("dem", Dem), ("dh", ctypes.c_void_p),
+                ("offpoly_sec", ctypes.c_double * 6), ("use_offpoly", ctypes.c_int32),
+                ("pad_", ctypes.c_int32),
+                ("ref_slc", ctypes.c_void_p), ("sec_slc", ctypes.c_void_p),
+                ("ref_deramped", ctypes.c_void_p), ("sec_deramped", ctypes.c_void_p),
+                ("ifg_ml", ctypes.c_void_p), ("pow_ref", ctypes.c_void_p),
+                ("pow_sec", ctypes.c_void_p), ("filt", ctypes.c_void_p),
+                ("coh", ctypes.c_void_p)]
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("interp", ctypes.c_int32), ("rl", ctypes.c_int32), ("cl", ctypes.c_int32),
+                ("alpha", ctypes.c_float), ("gold_mode", ctypes.c_int32),
+                ("coh_win", ctypes.c_int32), ("coh_fill", ctypes.c_int32),
+                ("coh_seed", ctypes.c_uint32), ("pair_block_x", ctypes.c_int32),
+                ("pair_blocks_per_cu", ctypes.c_int32), ("gold_blocks_per_cu", ctypes.c_int32),
+                ("stream_blocks_per_cu", ctypes.c_int32)]
+
+
+LAUNCH_FIELDS = ("pair_block_x", "pair_blocks_per_cu", "gold_blocks_per_cu",
+                 "stream_blocks_per_cu")
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _I64 = ctypes.c_int64
@@ -103,8 +127,15 @@ SIGNATURES = {
     "s1ifg_goldstein": [_P, _P, _I, _I, _I, _F, _I, _P],
     "s1ifg_pair": [_P, _P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P, _P, _I,
                    _I, _I, _P, _P, _P],
+    "s1ifg_batch_create": [_P, ctypes.POINTER(PairTask), _I, ctypes.POINTER(BatchOpts),
+                           ctypes.POINTER(_P)],
+    "s1ifg_batch_destroy": [_P],
+    "s1ifg_batch_run": [_P, _I],
+    "s1ifg_batch_bytes": [_P, ctypes.POINTER(_I64)],
+    "s1ifg_batch_get_opts": [_P, ctypes.POINTER(BatchOpts)],
+    "s1ifg_batch_set_launch": [_P, ctypes.POINTER(BatchOpts)],
 }
-_VOID = {"s1ifg_ctx_destroy", "s1ifg_geom_destroy"}
+_VOID = {"s1ifg_ctx_destroy", "s1ifg_geom_destroy", "s1ifg_batch_destroy"}
 _STR = {"s1ifg_last_error"}
 
 _lib = None

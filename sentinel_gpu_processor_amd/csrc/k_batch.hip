@@ -1,0 +1,566 @@
+// Batched burst pairs: every stage of the pair chain runs once for all pairs
+// of the batch, as a persistent kernel walking a flattened work-item space
+// (rows, tiles or patches of every pair, pairs of uneven size back to back).
+// A work item is mapped to its pair by a binary search over the stage's
+// prefix array.  The per-item device code is the one the single-pair entry
+// points run (geom_dev.h, gold_dev.h, gold_mfma_dev.h, ifg_dev.h), so each
+// pair's products are bit-identical to those entry points.
+//
+// The launch sequence can be captured once into a HIP graph (on the batch's
+// own stream) and replayed: every kernel writes all of its outputs, nothing
+// is cleared with memset, and the raw bursts are never written.
+#include <cstring>
+#include <vector>
+
+#include "geom_dev.h"
+#include "gold_dev.h"
+#include "gold_mfma_dev.h"
+#include "ifg_dev.h"
+
+namespace s1ifg {
+
+int goldstein_grid(int nrow, int ncol, int *pi_n, int *pj_n);
+
+namespace {
+
+// Descriptors and prefix arrays are read through the constant address space
+// (scalar loads into SGPRs, not vector registers).
+#define S1C __attribute__((address_space(4)))
+
+struct PairDev {
+  const S1C GeomDev *gr, *gs;
+  OffPoly op;
+  GridDev grid;
+  DemDev dem;
+  const float *dh;
+  const float2 *ref_in, *sec_in;
+  float2 *ref_d, *sec_d;
+  RowTab *rows;
+  ColTab *cols;
+  float2 *ifg;
+  float *pr, *ps;
+  float2 *patches, *filt;
+  float *coh;
+  int nr_ml, nc_ml, pi_n, pj_n;
+};
+
+enum { kStTables, kStDeramp, kStPair, kStGold, kStGather, kStCoh, kNumStages };
+
+__device__ __forceinline__ const GeomDev &geo(const S1C GeomDev *g) { return *(const GeomDev *)g; }
+
+// Largest p in [0, n) with off[p] <= idx, for 0 <= idx < off[n].
+__device__ __forceinline__ int find_item(const S1C long *__restrict__ off, int n, long idx) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (off[mid] <= idx)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Grid tables of every pair; one thread per table entry.
+__global__ void kb_tables(const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n) {
+  const long total = off[n];
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long)gridDim.x * blockDim.x) {
+    int p = find_item(off, n, t);
+    const PairDev &d = *(const PairDev *)&P[p];
+    grid_table_entry(d.grid, d.dem, (int)(t - off[p]), d.rows, d.cols);
+  }
+}
+
+// Deramp of every burst (image 2p = reference of pair p, 2p + 1 = secondary);
+// one block per burst line, the element code of k_deramp.
+__global__ __launch_bounds__(256) void kb_deramp(const S1C PairDev *__restrict__ P,
+                                                 const S1C long *__restrict__ off, int nimg) {
+  const long total = off[nimg];
+  for (long t = blockIdx.x; t < total; t += gridDim.x) {
+    const int k = find_item(off, nimg, t);
+    const PairDev &d = *(const PairDev *)&P[k >> 1];
+    const GeomDev &g = geo((k & 1) ? d.gs : d.gr);
+    const float2 *in = (k & 1) ? d.sec_in : d.ref_in;
+    float2 *out = (k & 1) ? d.sec_d : d.ref_d;
+    const int i = (int)(t - off[k]);
+    const int N = g.samples;
+    const double eta = -g.lines * g.ati / 2. + i * g.ati;
+    const size_t rowoff = (size_t)i * N;
+    const bool vec = (N % 2 == 0) && (((uintptr_t)in | (uintptr_t)out) % 16 == 0);
+    const float4 *in2 = (const float4 *)in;
+    float4 *out2 = (float4 *)out;
+    for (int jj = threadIdx.x; jj < (N + 1) / 2; jj += blockDim.x) {
+      int j = 2 * jj;
+      if (vec) {
+        float4 a = in2[(rowoff >> 1) + jj];
+        float2 o0 = deramp_elem(eta, g.kt[j], g.etaref[j], make_float2(a.x, a.y));
+        float2 o1 = deramp_elem(eta, g.kt[j + 1], g.etaref[j + 1], make_float2(a.z, a.w));
+        out2[(rowoff >> 1) + jj] = make_float4(o0.x, o0.y, o1.x, o1.y);
+      } else {
+        for (int q = j; q < j + 2 && q < N; ++q)
+          out[rowoff + q] = deramp_elem(eta, g.kt[q], g.etaref[q], in[rowoff + q]);
+      }
+    }
+  }
+}
+
+// Fused pair kernel over tiles of blockDim.x x blockDim.y multilooked pixels.
+template <int INTERP>
+__global__ __launch_bounds__(256) void kb_pair(const S1C PairDev *__restrict__ P,
+                                               const S1C long *__restrict__ off, int n, int rl,
+                                               int cl) {
+  const long total = off[n];
+  for (long t = blockIdx.x; t < total; t += gridDim.x) {
+    const int p = find_item(off, n, t);
+    const PairDev &d = *(const PairDev *)&P[p];
+    const long lt = t - off[p];
+    const int tiles_x = (d.nc_ml + blockDim.x - 1) / blockDim.x;
+    const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
+    const int cm = tx * blockDim.x + threadIdx.x, rm = ty * blockDim.y + threadIdx.y;
+    if (rm < d.nr_ml && cm < d.nc_ml)
+      pair_ml_pixel<INTERP>(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
+                            d.sec_d, rl, cl, rm, cm, d.nc_ml, d.ifg, d.pr, d.ps);
+  }
+}
+
+constexpr int kGoldWaves = 4;
+
+// Goldstein patch pass on MFMA: persistent wavefronts over the patches of
+// every pair, the next patch prefetched (no block barriers).
+__global__ __launch_bounds__(256) void kb_gold_mfma(const S1C PairDev *__restrict__ P,
+                                                    const S1C long *__restrict__ off, int n,
+                                                    float alpha) {
+  const int lane = threadIdx.x & 63;
+  const int h = lane >> 5, nn = lane & 31;
+  const long total = off[n];
+  const long nwaves = (long)gridDim.x * kGoldWaves;
+  long patch = (long)blockIdx.x * kGoldWaves + (threadIdx.x >> 6);
+  if (patch >= total) return;
+  mf::Consts K;
+  mf::build_consts(h, nn, K);
+  int p = find_item(off, n, patch);
+  float2 nxt[16];
+  {
+    const PairDev &d = *(const PairDev *)&P[p];
+    mf::load_patch(d.ifg, d.nr_ml, d.nc_ml, d.pj_n, patch - off[p], h, nn, nxt);
+  }
+  for (; patch < total; patch += nwaves) {
+    float2 x[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x[r] = nxt[r];
+    const int pc = p;
+    const long next = patch + nwaves;
+    if (next < total) {
+      p = find_item(off, n, next);
+      const PairDev &d = *(const PairDev *)&P[p];
+      mf::load_patch(d.ifg, d.nr_ml, d.nc_ml, d.pj_n, next - off[p], h, nn, nxt);
+    }
+    const PairDev &dc = *(const PairDev *)&P[pc];
+    mf::filter_patch(K, x, h, nn, alpha, dc.patches + (size_t)(patch - off[pc]) * (mf::kN * mf::kN));
+  }
+}
+
+constexpr int kGoldGroups = 4;
+
+// FP32 Goldstein patch pass: 4 groups of 32 lanes per block, one patch each.
+__global__ __launch_bounds__(128) void kb_gold_fp32(const S1C PairDev *__restrict__ P,
+                                                    const S1C long *__restrict__ off, int n,
+                                                    float alpha) {
+  __shared__ float2 T[kGoldGroups][kGoldN * kGoldPad];
+  __shared__ float M[kGoldGroups][kGoldN * kGoldPad];
+  const int lane = threadIdx.x & 31;
+  const int sub = threadIdx.x >> 5;
+  const long total = off[n];
+  for (long base = (long)blockIdx.x * kGoldGroups; base < total;
+       base += (long)gridDim.x * kGoldGroups) {
+    const long patch = base + sub;
+    const bool active = patch < total;
+    const int p = find_item(off, n, active ? patch : base);
+    const PairDev &d = *(const PairDev *)&P[p];
+    const long lp = active ? patch - off[p] : 0;
+    fp32_patch_group(d.ifg, d.nr_ml, d.nc_ml, d.pj_n, lp, active, alpha,
+                     d.patches + (size_t)lp * (kGoldN * kGoldN), T[sub], M[sub], lane);
+    __syncthreads();
+  }
+}
+
+// Overlap-add gather over 64 x 4 tiles of every filtered image.
+__global__ __launch_bounds__(256) void kb_gather(const S1C PairDev *__restrict__ P,
+                                                 const S1C long *__restrict__ off, int n) {
+  const long total = off[n];
+  for (long t = blockIdx.x; t < total; t += gridDim.x) {
+    const int p = find_item(off, n, t);
+    const PairDev &d = *(const PairDev *)&P[p];
+    const long lt = t - off[p];
+    const int tiles_x = (d.nc_ml + 63) / 64;
+    const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
+    const int x = tx * 64 + threadIdx.x, y = ty * 4 + threadIdx.y;
+    if (x < d.nc_ml && y < d.nr_ml)
+      d.filt[(size_t)y * d.nc_ml + x] =
+          gold_gather_pixel(d.patches, d.nr_ml, d.nc_ml, d.pi_n, d.pj_n, y, x);
+  }
+}
+
+// Coherence over 64 x 4 tiles: LDS-tiled window sums (win >= 3) or the
+// look-cell estimator (win = 1).
+__global__ __launch_bounds__(256) void kb_coherence(const S1C PairDev *__restrict__ P,
+                                                    const S1C long *__restrict__ off, int n,
+                                                    int win, int fill, uint32_t seed) {
+  __shared__ CohTile L;
+  const long total = off[n];
+  for (long t = blockIdx.x; t < total; t += gridDim.x) {
+    const int p = find_item(off, n, t);
+    const PairDev &d = *(const PairDev *)&P[p];
+    const long lt = t - off[p];
+    const int tiles_x = (d.nc_ml + kCohTX - 1) / kCohTX;
+    const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
+    if (win == 1) {
+      const int c = tx * kCohTX + threadIdx.x, r = ty * kCohTY + threadIdx.y;
+      if (r < d.nr_ml && c < d.nc_ml) {
+        size_t o = (size_t)r * d.nc_ml + c;
+        d.coh[o] = coh_fill(coherence_cell(d.ifg[o], d.pr[o], d.ps[o]), fill, seed, o);
+      }
+    } else {
+      coherence_tile(d.ifg, d.pr, d.ps, d.nr_ml, d.nc_ml, win, fill, seed, d.coh, tx, ty, L);
+    }
+  }
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+}  // namespace s1ifg
+
+using namespace s1ifg;
+
+struct s1ifg_batch {
+  s1ifg_ctx *ctx;
+  int n;
+  s1ifg_batch_opts opts;
+  std::vector<long> off[kNumStages];  // host copies of the prefix arrays
+  std::vector<PairDev> pairs;
+  void *d_buf = nullptr;
+  size_t bytes = 0;
+  PairDev *d_pairs = nullptr;
+  long *d_off[kNumStages] = {};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int fill_defaults(s1ifg_batch_opts *o) {
+  if (o->pair_block_x == 0) o->pair_block_x = 64;
+  if (o->pair_blocks_per_cu == 0) o->pair_blocks_per_cu = 8;
+  if (o->gold_blocks_per_cu == 0) o->gold_blocks_per_cu = 2;
+  if (o->stream_blocks_per_cu == 0) o->stream_blocks_per_cu = 8;
+  const int bx = o->pair_block_x;
+  if (bx != 16 && bx != 32 && bx != 64 && bx != 128 && bx != 256)
+    return set_error(S1IFG_EINVAL, "pair_block_x %d (16, 32, 64, 128 or 256)", bx);
+  if (o->pair_blocks_per_cu < 1 || o->pair_blocks_per_cu > 64 || o->gold_blocks_per_cu < 1 ||
+      o->gold_blocks_per_cu > 64 || o->stream_blocks_per_cu < 1 || o->stream_blocks_per_cu > 64)
+    return set_error(S1IFG_EINVAL, "blocks per CU must be in [1, 64]");
+  return S1IFG_OK;
+}
+
+// Pair-kernel tile prefix for the current tile width.
+void pair_prefix(s1ifg_batch *b) {
+  const int bx = b->opts.pair_block_x, by = 256 / bx;
+  std::vector<long> &o = b->off[kStPair];
+  o.assign(b->n + 1, 0);
+  for (int p = 0; p < b->n; ++p) {
+    const PairDev &d = b->pairs[p];
+    long tiles = (long)((d.nc_ml + bx - 1) / bx) * ((d.nr_ml + by - 1) / by);
+    o[p + 1] = o[p] + tiles;
+  }
+}
+
+void drop_graph(s1ifg_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+unsigned grid_of(long items, long per_block, long cap) {
+  long want = (items + per_block - 1) / per_block;
+  return (unsigned)(want < cap ? want : cap);
+}
+
+// The launch sequence, enqueued on stream s.
+int enqueue(s1ifg_batch *b, hipStream_t s) {
+  const s1ifg_batch_opts &o = b->opts;
+  const long ncu = device_cu_count(b->ctx->device);
+  const int n = b->n;
+  const S1C long *off[kNumStages];
+  for (int st = 0; st < kNumStages; ++st) off[st] = (const S1C long *)b->d_off[st];
+  const S1C PairDev *pairs = (const S1C PairDev *)b->d_pairs;
+  auto total = [&](int st) { return b->off[st].back(); };
+  if (long t = total(kStTables)) {
+    hipLaunchKernelGGL(kb_tables, dim3(grid_of(t, 256, ncu * o.stream_blocks_per_cu)), dim3(256),
+                       0, s, pairs, off[kStTables], n);
+    S1_LAUNCH_CHECK("kb_tables");
+  }
+  if (long t = total(kStDeramp)) {
+    hipLaunchKernelGGL(kb_deramp, dim3(grid_of(t, 1, ncu * o.stream_blocks_per_cu)), dim3(256), 0,
+                       s, pairs, off[kStDeramp], 2 * n);
+    S1_LAUNCH_CHECK("kb_deramp");
+  }
+  if (long t = total(kStPair)) {
+    dim3 blk(o.pair_block_x, 256 / o.pair_block_x);
+    dim3 grd(grid_of(t, 1, ncu * o.pair_blocks_per_cu));
+    if (o.interp == S1IFG_INTERP_BILINEAR)
+      hipLaunchKernelGGL(kb_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, s, pairs, off[kStPair],
+                         n, o.rl, o.cl);
+    else
+      hipLaunchKernelGGL(kb_pair<S1IFG_INTERP_SINC>, grd, blk, 0, s, pairs, off[kStPair], n,
+                         o.rl, o.cl);
+    S1_LAUNCH_CHECK("kb_pair");
+  }
+  if (long t = total(kStGold)) {
+    if (o.gold_mode == S1IFG_GOLD_FP16_SPLIT)
+      hipLaunchKernelGGL(kb_gold_mfma, dim3(grid_of(t, kGoldWaves, ncu * o.gold_blocks_per_cu)),
+                         dim3(64 * kGoldWaves), 0, s, pairs, off[kStGold], n, o.alpha);
+    else
+      hipLaunchKernelGGL(kb_gold_fp32, dim3(grid_of(t, kGoldGroups, ncu * 8 * o.gold_blocks_per_cu)),
+                         dim3(32 * kGoldGroups), 0, s, pairs, off[kStGold], n, o.alpha);
+    S1_LAUNCH_CHECK("kb_gold");
+  }
+  if (long t = total(kStGather)) {
+    hipLaunchKernelGGL(kb_gather, dim3(grid_of(t, 1, ncu * o.stream_blocks_per_cu)), dim3(64, 4),
+                       0, s, pairs, off[kStGather], n);
+    S1_LAUNCH_CHECK("kb_gather");
+  }
+  if (long t = total(kStCoh)) {
+    hipLaunchKernelGGL(kb_coherence, dim3(grid_of(t, 1, ncu * o.stream_blocks_per_cu)),
+                       dim3(kCohTX, kCohTY), 0, s, pairs, off[kStCoh], n, o.coh_win,
+                       o.coh_fill, o.coh_seed);
+    S1_LAUNCH_CHECK("kb_coherence");
+  }
+  return S1IFG_OK;
+}
+
+int upload_prefix(s1ifg_batch *b, int st) {
+  S1_HIP(hipMemcpy(b->d_off[st], b->off[st].data(), b->off[st].size() * sizeof(long),
+                   hipMemcpyHostToDevice));
+  return S1IFG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
+                       const s1ifg_batch_opts *opts, s1ifg_batch **out) {
+  if (!out) return set_error(S1IFG_EINVAL, "null output pointer");
+  *out = nullptr;
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!tasks || n < 1 || !opts) return set_error(S1IFG_EINVAL, "empty batch");
+  s1ifg_batch_opts o = *opts;
+  if ((rc = fill_defaults(&o))) return rc;
+  if (o.interp != S1IFG_INTERP_BILINEAR && o.interp != S1IFG_INTERP_SINC)
+    return set_error(S1IFG_EINVAL, "interp %d", o.interp);
+  if (o.rl < 1 || o.cl < 1) return set_error(S1IFG_EINVAL, "looks %d x %d", o.rl, o.cl);
+  if (o.gold_mode != S1IFG_GOLD_FP32 && o.gold_mode != S1IFG_GOLD_FP16_SPLIT)
+    return set_error(S1IFG_EINVAL, "gold_mode %d", o.gold_mode);
+  if (!(o.alpha >= 0.f) || !std::isfinite(o.alpha)) return set_error(S1IFG_EINVAL, "alpha %g", o.alpha);
+  if (o.coh_win < 1 || o.coh_win % 2 == 0)
+    return set_error(S1IFG_EINVAL, "coherence window %d (odd, >= 1)", o.coh_win);
+  if (o.coh_win / 2 > kCohMaxHalo)
+    return set_error(S1IFG_ERANGE, "batched coherence window at most %d", 2 * kCohMaxHalo + 1);
+  if (o.coh_fill < S1IFG_COH_FILL_CLAMP || o.coh_fill > S1IFG_COH_FILL_HASH)
+    return set_error(S1IFG_EINVAL, "coherence fill %d", o.coh_fill);
+
+  std::vector<PairDev> pairs(n);
+  size_t tab_bytes = 0, patch_bytes = 0;
+  std::vector<size_t> tab_off(n), patch_off(n);
+  for (int p = 0; p < n; ++p) {
+    const s1ifg_pair_task &t = tasks[p];
+    PairDev &d = pairs[p];
+    std::memset(&d, 0, sizeof(d));
+    if ((rc = check_geom(ctx, t.g_ref)) || (rc = check_geom(ctx, t.g_sec))) return rc;
+    if ((rc = make_grid(&t.grid, &d.grid)) || (rc = make_dem(&t.dem, &d.dem))) return rc;
+    if ((rc = make_offpoly(t.use_offpoly ? t.offpoly_sec : nullptr, &d.op))) return rc;
+    if (!t.ref_slc || !t.sec_slc || !t.ref_deramped || !t.sec_deramped || !t.ifg_ml)
+      return set_error(S1IFG_EINVAL, "pair %d: null data pointer", p);
+    if (t.ref_deramped == t.ref_slc || t.sec_deramped == t.sec_slc ||
+        t.ref_deramped == t.sec_deramped)
+      return set_error(S1IFG_EINVAL, "pair %d: deramped bursts must not alias the raw bursts", p);
+    d.gr = (const S1C GeomDev *)t.g_ref->d_dev;
+    d.gs = (const S1C GeomDev *)t.g_sec->d_dev;
+    d.dh = t.dh;
+    d.ref_in = (const float2 *)t.ref_slc;
+    d.sec_in = (const float2 *)t.sec_slc;
+    d.ref_d = (float2 *)t.ref_deramped;
+    d.sec_d = (float2 *)t.sec_deramped;
+    d.ifg = (float2 *)t.ifg_ml;
+    d.pr = t.pow_ref;
+    d.ps = t.pow_sec;
+    d.filt = (float2 *)t.filt;
+    d.coh = t.coh;
+    d.nr_ml = d.grid.nrow / o.rl;
+    d.nc_ml = d.grid.ncol / o.cl;
+    if (d.nr_ml < 1 || d.nc_ml < 1)
+      return set_error(S1IFG_EINVAL, "pair %d: grid smaller than one look cell", p);
+    if (d.coh && (!d.pr || !d.ps))
+      return set_error(S1IFG_EINVAL, "pair %d: coherence needs both power images", p);
+    if (d.filt) {
+      if (d.nr_ml < kGoldN || d.nc_ml < kGoldN)
+        return set_error(S1IFG_EINVAL, "pair %d: multilooked image %d x %d smaller than the %d-pixel window",
+                         p, d.nr_ml, d.nc_ml, kGoldN);
+      if (d.filt == d.ifg) return set_error(S1IFG_EINVAL, "pair %d: in-place filtering is not supported", p);
+      goldstein_grid(d.nr_ml, d.nc_ml, &d.pi_n, &d.pj_n);
+      patch_off[p] = patch_bytes;
+      patch_bytes += align256((size_t)d.pi_n * d.pj_n * kGoldN * kGoldN * sizeof(float2));
+    }
+    tab_off[p] = tab_bytes;
+    tab_bytes += align256(sizeof(RowTab) * (size_t)d.grid.nrow) +
+                 align256(sizeof(ColTab) * (size_t)d.grid.ncol);
+  }
+
+  s1ifg_batch *b = new s1ifg_batch();
+  b->ctx = ctx;
+  b->n = n;
+  b->opts = o;
+  b->pairs = pairs;
+  // prefix arrays
+  for (int st = 0; st < kNumStages; ++st) b->off[st].assign(st == kStDeramp ? 2 * n + 1 : n + 1, 0);
+  for (int p = 0; p < n; ++p) {
+    const PairDev &d = b->pairs[p];
+    const s1ifg_pair_task &t = tasks[p];
+    long tab = d.grid.nrow > d.grid.ncol ? d.grid.nrow : d.grid.ncol;
+    b->off[kStTables][p + 1] = b->off[kStTables][p] + tab;
+    b->off[kStDeramp][2 * p + 1] = b->off[kStDeramp][2 * p] + t.g_ref->dev.lines;
+    b->off[kStDeramp][2 * p + 2] = b->off[kStDeramp][2 * p + 1] + t.g_sec->dev.lines;
+    b->off[kStGold][p + 1] = b->off[kStGold][p] + (d.filt ? (long)d.pi_n * d.pj_n : 0);
+    long gtiles = (long)((d.nc_ml + 63) / 64) * ((d.nr_ml + 3) / 4);
+    b->off[kStGather][p + 1] = b->off[kStGather][p] + (d.filt ? gtiles : 0);
+    long ctiles = (long)((d.nc_ml + kCohTX - 1) / kCohTX) * ((d.nr_ml + kCohTY - 1) / kCohTY);
+    b->off[kStCoh][p + 1] = b->off[kStCoh][p] + (d.coh ? ctiles : 0);
+  }
+  pair_prefix(b);
+
+  // one device buffer: descriptors | prefix arrays | grid tables | patches
+  size_t off_pairs = 0;
+  size_t sz = align256(sizeof(PairDev) * n);
+  size_t off_pref[kNumStages];
+  for (int st = 0; st < kNumStages; ++st) {
+    off_pref[st] = sz;
+    sz += align256(sizeof(long) * (2 * n + 1));
+  }
+  size_t off_tab = sz;
+  sz += tab_bytes;
+  size_t off_patch = sz;
+  sz += patch_bytes;
+  (void)hipSetDevice(ctx->device);
+  if (hipMalloc(&b->d_buf, sz) != hipSuccess) {
+    (void)hipGetLastError();
+    delete b;
+    return set_error(S1IFG_ENOMEM, "batch scratch of %zu bytes", sz);
+  }
+  b->bytes = sz;
+  char *base = (char *)b->d_buf;
+  b->d_pairs = (PairDev *)(base + off_pairs);
+  for (int st = 0; st < kNumStages; ++st) b->d_off[st] = (long *)(base + off_pref[st]);
+  for (int p = 0; p < n; ++p) {
+    PairDev &d = b->pairs[p];
+    d.rows = (RowTab *)(base + off_tab + tab_off[p]);
+    d.cols = (ColTab *)(base + off_tab + tab_off[p] + align256(sizeof(RowTab) * (size_t)d.grid.nrow));
+    d.patches = d.filt ? (float2 *)(base + off_patch + patch_off[p]) : nullptr;
+  }
+  int err = S1IFG_OK;
+  if (hipMemcpy(b->d_pairs, b->pairs.data(), sizeof(PairDev) * n, hipMemcpyHostToDevice) != hipSuccess)
+    err = set_error(S1IFG_EHIP, "descriptor upload failed");
+  for (int st = 0; st < kNumStages && !err; ++st) err = upload_prefix(b, st);
+  if (!err && (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+               hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
+               hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess))
+    err = set_error(S1IFG_EHIP, "batch stream / event creation failed");
+  if (err) {
+    s1ifg_batch_destroy(b);
+    return err;
+  }
+  *out = b;
+  return S1IFG_OK;
+}
+
+void s1ifg_batch_destroy(s1ifg_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+  if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->d_buf) (void)hipFree(b->d_buf);
+  delete b;
+}
+
+int s1ifg_batch_run(s1ifg_batch *b, int use_graph) {
+  if (!b) return set_error(S1IFG_EINVAL, "null batch");
+  int rc = check_ctx(b->ctx);
+  if (rc) return rc;
+  if (!use_graph) return enqueue(b, b->ctx->stream);
+  if (!b->exec) {
+    S1_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue(b, b->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_graph(b);
+      return hip_error(e, "hipGraphInstantiate");
+    }
+  }
+  // the graph runs on the batch stream, ordered after and before the
+  // context stream's work
+  S1_HIP(hipEventRecord(b->ev_in, b->ctx->stream));
+  S1_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0));
+  S1_HIP(hipGraphLaunch(b->exec, b->stream));
+  S1_HIP(hipEventRecord(b->ev_out, b->stream));
+  S1_HIP(hipStreamWaitEvent(b->ctx->stream, b->ev_out, 0));
+  return S1IFG_OK;
+}
+
+int s1ifg_batch_bytes(const s1ifg_batch *b, int64_t *bytes) {
+  if (!b || !bytes) return set_error(S1IFG_EINVAL, "null argument");
+  *bytes = (int64_t)b->bytes;
+  return S1IFG_OK;
+}
+
+int s1ifg_batch_get_opts(const s1ifg_batch *b, s1ifg_batch_opts *opts) {
+  if (!b || !opts) return set_error(S1IFG_EINVAL, "null argument");
+  *opts = b->opts;
+  return S1IFG_OK;
+}
+
+int s1ifg_batch_set_launch(s1ifg_batch *b, const s1ifg_batch_opts *opts) {
+  if (!b || !opts) return set_error(S1IFG_EINVAL, "null argument");
+  s1ifg_batch_opts o = b->opts;
+  o.pair_block_x = opts->pair_block_x;
+  o.pair_blocks_per_cu = opts->pair_blocks_per_cu;
+  o.gold_blocks_per_cu = opts->gold_blocks_per_cu;
+  o.stream_blocks_per_cu = opts->stream_blocks_per_cu;
+  int rc = fill_defaults(&o);
+  if (rc) return rc;
+  if ((rc = check_ctx(b->ctx))) return rc;
+  // the prefix array is read by kernels already enqueued
+  S1_HIP(hipStreamSynchronize(b->ctx->stream));
+  S1_HIP(hipStreamSynchronize(b->stream));
+  drop_graph(b);
+  b->opts = o;
+  pair_prefix(b);
+  return upload_prefix(b, kStPair);
+}
+
+}  // extern "C"

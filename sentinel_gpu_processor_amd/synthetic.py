@@ -254,7 +254,44 @@ CONFIGS = {
                offset=None),
     "c2": dict(lines=1500, samples=25000, dem=(2000, 3000), dem_kind="fractal", hmax=2500.0,
                offset=(0.3, 0.05)),
+    # 27 bursts (3 subswaths x 9), sizes drawn from the seed
+    "c3": dict(n_sub=3, n_burst=9, lines=(1450, 1530), samples=(20000, 25500), dem=(2000, 3000),
+               dem_kind="fractal", hmax=2500.0, offset=(0.3, 0.05)),
 }
+
+
+def iw_sizes(seed=0, n_sub=3, n_burst=9, lines=(1450, 1530), samples=(20000, 25500)):
+    """Burst sizes of a synthetic IW pair: n_sub x n_burst sizes drawn
+    uniformly (integers, bounds inclusive) from the seed."""
+    rng = np.random.default_rng(seed)
+    n = n_sub * n_burst
+    L = rng.integers(lines[0], lines[1] + 1, n)
+    N = rng.integers(samples[0], samples[1] + 1, n)
+    return [(int(a), int(b)) for a, b in zip(L, N)]
+
+
+def make_iw(seed=0, n_sub=3, n_burst=9, lines=(1450, 1530), samples=(20000, 25500),
+            dem_shape=(2000, 3000), dem_kind="fractal", hmax=2500.0, misreg=(0.3, 0.05),
+            fidelity="fast", device=None):
+    """Synthetic IW pair: n_sub subswaths x n_burst bursts of uneven size.
+    Burst k of subswath s is centred 0.17 deg further along track than burst
+    k-1 (about one burst length) and subswath s looks 50 km further in
+    range than s-1; each burst pair has its own DEM window and seed."""
+    sizes = iw_sizes(seed, n_sub, n_burst, lines, samples)
+    out = []
+    for s in range(n_sub):
+        for k in range(n_burst):
+            L, N = sizes[s * n_burst + k]
+            sc = make_scene(lines=L, samples=N, dem_shape=dem_shape, dem_kind=dem_kind,
+                            hmax=hmax, seed=seed * 1000 + s * n_burst + k, misreg=misreg,
+                            misreg_ramp=(0.2, 0.1) if misreg else (0.0, 0.0),
+                            fidelity=fidelity, device=device,
+                            lat_c=34.0 + 0.17 * (k - n_burst // 2),
+                            look_range=800.0e3 + 50.0e3 * s,
+                            t_center=43200.0 + 2.75 * k)
+            sc["subswath"], sc["burst"] = s + 1, k + 1
+            out.append(sc)
+    return out
 
 
 def make_scene(lines=512, samples=2048, dem_shape=(256, 256), dem_kind="hill", hmax=1500.0,

@@ -19,6 +19,9 @@ def scene(name="small", seed=0):
                             misreg=(0.3, 0.05), misreg_ramp=(0.2, 0.1))
     if name == "odd":
         return S.make_scene(lines=97, samples=301, dem_shape=(40, 50), hmax=200.0, seed=seed)
+    if name == "wide":
+        return S.make_scene(lines=150, samples=640, dem_shape=(48, 80), hmax=250.0, seed=seed,
+                            misreg=(-0.2, 0.1))
     if name == "c1":
         return S.make_scene(**_c1_kwargs(), seed=seed)
     raise KeyError(name)

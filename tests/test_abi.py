@@ -2,13 +2,16 @@
 (no compute calls: no GPU needed)."""
 
 import os
+import pathlib
 import re
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
 
 
 def declared_symbols():
-    src = open(os.path.join(ROOT, "include", "s1ifg.h")).read()
+    src = open(ROOT / "include" / "s1ifg.h").read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(s1ifg_[a-z0-9_]+)\s*\(", src)))
 
@@ -52,3 +55,35 @@ def test_struct_layouts():
     assert ctypes.sizeof(N.Grid) == 40
     assert N.Dem.lat0.offset == 24
     assert ctypes.sizeof(N.Dem) == 56
+
+
+def test_struct_layouts_match_c_compiler(tmp_path):
+    """Every ctypes structure has the size and field offsets that a C
+    compiler gives the header's structures."""
+    import ctypes
+    import shutil
+    import subprocess
+    from sentinel_gpu_processor_amd import _native as N
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"s1ifg_orbit": N.Orbit, "s1ifg_burst": N.Burst, "s1ifg_grid": N.Grid,
+               "s1ifg_dem": N.Dem, "s1ifg_pair_task": N.PairTask,
+               "s1ifg_batch_opts": N.BatchOpts}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "s1ifg.h"',
+             "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f, _t in cls._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        cname, field, value = line.split()
+        cls = structs[cname]
+        got = ctypes.sizeof(cls) if field == "sizeof" else getattr(cls, field).offset
+        assert got == int(value), (cname, field, got, value)
