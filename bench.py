@@ -49,6 +49,9 @@ def parse():
     p.add_argument("--gold-mode", default="auto", choices=["auto", "fp32", "fp16_split"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", default="c1")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also run a plain 1 GiB device copy (3 launches) whose HBM "
+                        "counters scale the path's FETCH_SIZE / WRITE_SIZE readings")
     return p.parse_args()
 
 
@@ -341,6 +344,18 @@ class StageEvents:
         return {k: v / steps for k, v in tot.items()}
 
 
+def calibrate_copy(nbytes=1 << 30, reps=3):
+    """A device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    per dispatch is the counters' reading of nbytes moved."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[bench] calibration copy: {reps} x {nbytes} bytes", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(sample):
     """FP64 NumPy oracle chain on a bounded sample (single process)."""
     import numpy as np
@@ -409,6 +424,8 @@ def main():
     roof = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK / 1e9,
                 unit="GB/s", frac=round(achieved * 1e9 / HBM_PEAK, 4), traffic=None,
                 alg_bytes_per_launch=sb[dom])
+    if args.calibrate_copy:
+        calibrate_copy()
     iw = None
     if not args.no_iw:
         try:
