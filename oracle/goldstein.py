@@ -7,9 +7,12 @@ is the unnormalised e^{+i} DFT = numpy.fft.ifft2 * n^2; the reference rescales
 by 1/n^2 in the overlap-add (goldstein.cu:267).
 
 Parity pins (tests/test_oracle_goldstein.py): alpha = 0 is the identity where
-the weight sum is positive; per-patch FFT against a brute-force DFT; the
-smoothing table values; constant input; zero patches; patch-grid counts and
-clamping against a direct loop restatement.
+the weight sum is positive; plane waves keep their phase; the filter commutes
+with complex scaling; the smoothing is the circular correlation of |Z| with the
+table (impulse responses); table symmetry and sum; patch-grid counts and
+clamping; additive triangular weights; zero patches stay zero.  The FFT itself
+is numpy.fft (parity with cuFFT unpinned: no reference test or vector holds
+cuFFT output).
 
 Deliberate departures (masked in parity): the reference never zeroes its
 output buffer (goldstein.cu:484), so pixels with weight sum 0 hold garbage
