@@ -263,7 +263,7 @@ def test_pair_matches_oracle(ctx, interp, name):
     cg = np.abs(ifg) / np.sqrt(pr * ps + 1e-30)
     cw = np.abs(oc["ifg"]) / np.sqrt(oc["p1"] * oc["p2"] + 1e-30)
     assert np.max(np.abs(cg[m] - cw[m])) < 1e-4
-    # fused path equals geocode + crossmul_looks bit for bit
+    # the fused path equals geocode + crossmul_looks to FP32 rounding
     o_r = K.geocode(ctx, gr, sc["grid"], dem_t, dem, dref, interp)
     o_s = K.geocode(ctx, gs, sc["grid"], dem_t, dem, dsec, interp, offpoly=sc["offpoly"])
     ifg2, pr2, ps2 = K.crossmul_looks(ctx, o_r, o_s, 1, 4)
