@@ -465,13 +465,15 @@ def offset_poly_shift(offpoly, rgoff, azoff, burst):
 
 
 def reproject(orbit, burst, setup, grid, heights, deramped, interp="bilinear",
-              offpoly=None, geo=None):
+              offpoly=None, geo=None, round_blend=True):
     """geo2rdr.cu:194-247 on a target grid; complex128 out (zeros where invalid).
 
     deramped: complex burst after deramp (complex64 as the kernel reads it).
     interp: "bilinear" (reference) or "sinc" (8-tap windowed sinc).
     offpoly: optional 6 offset-polynomial coefficients (sampling position and
     ramp only; the flattening phase keeps the geometric range).
+    round_blend=False keeps the bilinear range blends in FP64 (a smooth
+    function of the position, for finite-difference checks of the adjoint).
     """
     if geo is None:
         geo = geo2rdr_grid(orbit, burst, grid, heights)
@@ -522,10 +524,11 @@ def reproject(orbit, burst, setup, grid, heights, deramped, interp="bilinear",
         idx1 = inta * N + intr
         b1, b2, b3, b4 = z[idx1], z[idx1 + 1], z[idx1 + N], z[idx1 + N + 1]
         # geo2rdr.cu:232-235: the range blend is stored as float2
-        c1 = _to_cf32(b1.real * (1 - fracr) + b2.real * fracr,
-                      b1.imag * (1 - fracr) + b2.imag * fracr)
-        c2 = _to_cf32(b3.real * (1 - fracr) + b4.real * fracr,
-                      b3.imag * (1 - fracr) + b4.imag * fracr)
+        c1 = b1 * (1 - fracr) + b2 * fracr
+        c2 = b3 * (1 - fracr) + b4 * fracr
+        if round_blend:
+            c1 = _to_cf32(c1.real, c1.imag)
+            c2 = _to_cf32(c2.real, c2.imag)
         res = c1.astype(np.complex128) * (1 - fraca) + c2.astype(np.complex128) * fraca
     else:
         intr = np.floor(rg).astype(np.int64)

@@ -110,24 +110,28 @@ def coherence_lookcell(ifg_ml, amp1, amp2, fill="clamp"):
     return c
 
 
+def boxsum(x, win):
+    """Sum over the win x win window centred on each pixel, clipped at the
+    borders: direct sums of shifted copies, so all-zero windows give exactly
+    0 (a running-sum form leaves rounding residue there)."""
+    x = np.asarray(x)
+    h = win // 2
+    p = np.pad(x, h)
+    n0, n1 = x.shape
+    out = np.zeros_like(p[:n0, :n1])
+    for dy in range(win):
+        for dx in range(win):
+            out = out + p[dy:dy + n0, dx:dx + n1]
+    return out
+
+
 def coherence_window(ifg_ml, pow1, pow2, win):
     """Windowed coherence on the multilooked grid (no reference counterpart):
     |sum_W ifg| / sqrt(sum_W pow1 * sum_W pow2) over a win x win window
     clipped at the borders; 0 where the denominator is 0."""
-    h = win // 2
-
-    def boxsum(x):
-        x = np.asarray(x)
-        p = np.pad(x, h)
-        cs = np.cumsum(np.cumsum(p, axis=0), axis=1)
-        cs = np.pad(cs, ((1, 0), (1, 0)))
-        n0, n1 = x.shape
-        return (cs[win:win + n0, win:win + n1] - cs[:n0, win:win + n1]
-                - cs[win:win + n0, :n1] + cs[:n0, :n1])
-
-    si = boxsum(np.asarray(ifg_ml, np.complex128))
-    p1 = boxsum(np.asarray(pow1, np.float64))
-    p2 = boxsum(np.asarray(pow2, np.float64))
+    si = boxsum(np.asarray(ifg_ml, np.complex128), win)
+    p1 = boxsum(np.asarray(pow1, np.float64), win)
+    p2 = boxsum(np.asarray(pow2, np.float64), win)
     den = np.sqrt(p1 * p2)
     out = np.zeros(den.shape)
     ok = den > 0
