@@ -240,6 +240,31 @@ int s1ifg_pair(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
                const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl,
                s1ifg_cf32 *ifg_ml, float *pow_ref, float *pow_sec);
 
+/* ------------------------------------------------------------ adjoints */
+/* Adjoint of s1ifg_pair (no reference counterpart).  Given the gradients of
+ * a real loss with respect to the pair's outputs on the multilooked grid --
+ * g_ifg = dL/dRe ifg + i dL/dIm ifg, g_pow_ref, g_pow_sec (DEVICE; any may
+ * be NULL = 0) -- writes dL/d offpoly_sec (DEVICE double[6], may be NULL)
+ * and dL/d dh per target pixel (DEVICE float nrow*ncol, may be NULL; 0
+ * outside the look cells).  The forward is recomputed (geometry, sinc taps
+ * and their derivatives), nothing is stored.  Same arguments as s1ifg_pair
+ * for the forward part; offpoly_sec NULL = no offset polynomial (its
+ * gradient is still returned, at p = 0, if g_offpoly is given). */
+int s1ifg_pair_backward(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
+                        const s1ifg_grid *grid, const s1ifg_dem *dem, const float *dh,
+                        const double *offpoly_sec, const s1ifg_cf32 *ref_deramped,
+                        const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl,
+                        const s1ifg_cf32 *g_ifg, const float *g_pow_ref,
+                        const float *g_pow_sec, double *g_offpoly, float *g_dh);
+
+/* Adjoint of s1ifg_coherence with fill CLAMP / NAN / HASH (no gradient
+ * where the coherence was filled or is 0).  win = 1 needs all three
+ * outputs; win >= 3 writes the non-NULL ones. */
+int s1ifg_coherence_backward(s1ifg_ctx *ctx, const s1ifg_cf32 *ifg_ml,
+                             const float *pow_ref, const float *pow_sec, int nrow,
+                             int ncol, int win, const float *g_coh,
+                             s1ifg_cf32 *g_ifg, float *g_pow_ref, float *g_pow_sec);
+
 /* ------------------------------------------------------------ batches */
 /* One burst pair of a batch (all pointers DEVICE except the geometries).
  * The raw bursts are deramped into ref_deramped / sec_deramped, which must

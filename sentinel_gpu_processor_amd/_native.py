@@ -127,6 +127,9 @@ SIGNATURES = {
     "s1ifg_goldstein": [_P, _P, _I, _I, _I, _F, _I, _P],
     "s1ifg_pair": [_P, _P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P, _P, _I,
                    _I, _I, _P, _P, _P],
+    "s1ifg_pair_backward": [_P, _P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P,
+                            _P, _I, _I, _I, _P, _P, _P, _P, _P],
+    "s1ifg_coherence_backward": [_P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P],
     "s1ifg_batch_create": [_P, ctypes.POINTER(PairTask), _I, ctypes.POINTER(BatchOpts),
                            ctypes.POINTER(_P)],
     "s1ifg_batch_destroy": [_P],

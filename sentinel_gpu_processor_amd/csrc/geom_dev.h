@@ -51,9 +51,11 @@ __device__ __forceinline__ double rcp64(double x) {
 // (geo2rdr.cu:588-600).  The Newton derivative includes the dr.a term, so
 // the iteration converges quadratically to the same root; after the step
 // that meets the 5e-9 s criterion the orbit is advanced by a second-order
-// Taylor step instead of a further interpolation.  Returns dr = X - x(t).
+// Taylor step instead of a further interpolation.  Returns dr = X - x(t)
+// and, when asked, the orbit velocity and acceleration at t.
 __device__ __forceinline__ int zero_doppler(const GeomDev &g, const double X[3], double &t,
-                                            double dr[3]) {
+                                            double dr[3], double *vout = nullptr,
+                                            double *aout = nullptr) {
   double x[3] = {g.xmid[0], g.xmid[1], g.xmid[2]};
   double v[3] = {g.vmid[0], g.vmid[1], g.vmid[2]};
   double a[3] = {g.amid[0], g.amid[1], g.amid[2]};
@@ -69,6 +71,12 @@ __device__ __forceinline__ int zero_doppler(const GeomDev &g, const double X[3],
       dr[0] = d0 - v[0] * dt - a[0] * h2;
       dr[1] = d1 - v[1] * dt - a[1] * h2;
       dr[2] = d2 - v[2] * dt - a[2] * h2;
+      if (vout) {
+        for (int k = 0; k < 3; ++k) {
+          vout[k] = v[k] + a[k] * dt;
+          aout[k] = a[k];
+        }
+      }
       return it + 1;
     }
     orbit_eval(g.orb, t, x, v, a);
@@ -76,6 +84,12 @@ __device__ __forceinline__ int zero_doppler(const GeomDev &g, const double X[3],
   dr[0] = X[0] - x[0];
   dr[1] = X[1] - x[1];
   dr[2] = X[2] - x[2];
+  if (vout) {
+    for (int k = 0; k < 3; ++k) {
+      vout[k] = v[k];
+      aout[k] = a[k];
+    }
+  }
   return 52;
 }
 

@@ -1,0 +1,457 @@
+// Adjoint (reverse-mode derivative) of the fused burst pair and of the
+// coherence estimators.  No reference counterpart: the reference has no
+// gradient path.
+//
+// Pair: for a real loss L of (ifg_ml, pow_ref, pow_sec), given
+// gI = dL/dRe ifg + i dL/dIm ifg and the power gradients on the multilooked
+// grid, each target pixel of a look cell (n = rl * cl pixels) gets
+//   G_r = (conj(gI) z_s + 2 g_pr z_r) / n,   G_s = (gI z_r + 2 g_ps z_s) / n
+// (dL/dz in the same convention).  The forward of each burst is recomputed
+// (geometry, interpolation taps and their derivatives; nothing is stored)
+// and the chain runs through
+//   out = I(az, rg) e^{i phi},  phi = 4 pi R / lambda - reramp(rp, eta),
+//   rg = rgoff + drg,  az = azoff + daz,  rp = R + drg dmrg,
+//   eta = eta0 + (t - tstart) + daz dtaz,
+//   drg = p0 + p1 rgoff/N + p2 azoff/L,  daz = p3 + p4 rgoff/N + p5 azoff/L,
+// and for the height h: dt/dh = v.n / (|v|^2 - dr.a), dR/dh = dr.n / R with
+// n = dX/dh the ellipsoid normal.  Offset-polynomial gradients are reduced
+// per block and then across blocks in a fixed order (deterministic).
+//
+// Coherence: gamma = |A| / sqrt(B C) over the window (A, B, C window sums);
+// dgamma/dA = A / (|A| sqrt(B C)), dgamma/dB = -gamma / (2 B), and the
+// window sums are transposed by the same (clipped, symmetric) window sums.
+// Pixels whose coherence was filled (gamma > 1) or is 0 get no gradient.
+#include "geom_dev.h"
+#include "ifg_dev.h"
+
+using namespace s1ifg;
+
+namespace {
+
+// Sinc weights and their derivatives with respect to the fractional
+// offset f (the forward's 8-tap Hann-windowed sinc, in float).
+__device__ __forceinline__ void sinc_weights_d(float f, float w[8], float dw[8]) {
+  const float s = __builtin_amdgcn_sinf(0.5f * f), c = __builtin_amdgcn_cosf(0.5f * f);
+  const float s4 = __builtin_amdgcn_sinf(0.125f * f), c4 = __builtin_amdgcn_cosf(0.125f * f);
+  const float ck[8] = {-0.70710678f, 0.0f, 0.70710678f, 1.0f, 0.70710678f, 0.0f, -0.70710678f, -1.0f};
+  const float sk[8] = {-0.70710678f, -1.0f, -0.70710678f, 0.0f, 0.70710678f, 1.0f, 0.70710678f, 0.0f};
+  const float kPiF = 3.14159265358979f;
+  float h[8], dh[8], S = 0.f, dS = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    int k = i - 3;
+    float x = f - (float)k;
+    float sn = (k & 1) ? -s : s;   // sin(pi x)
+    float cs = (k & 1) ? -c : c;   // cos(pi x)
+    float sc, dsc;
+    if (x == 0.0f) {
+      sc = 1.0f;
+      dsc = 0.0f;
+    } else {
+      float ix = __builtin_amdgcn_rcpf(x);
+      sc = sn * ix * (1.0f / kPiF);
+      dsc = (cs - sc) * ix;
+    }
+    float cw = c4 * ck[i] + s4 * sk[i];    // cos(pi x / 4)
+    float sw = s4 * ck[i] - c4 * sk[i];    // sin(pi x / 4)
+    float win = 0.5f * (1.0f + cw);
+    float dwin = -0.125f * kPiF * sw;
+    h[i] = sc * win;
+    dh[i] = dsc * win + sc * dwin;
+    S += h[i];
+    dS += dh[i];
+  }
+  float inv = 1.0f / S;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    w[i] = h[i] * inv;
+    dw[i] = (dh[i] - w[i] * dS) * inv;
+  }
+}
+
+// One burst at one target pixel: value and the partials the adjoint needs.
+struct BurstAdj {
+  float2 z, dzr, dza;       // out, d out / d rg, d out / d az (rotation included)
+  double R_rp, R_eta;       // d reramp / d rp, d reramp / d eta
+  double dt_dh, drng_dh;
+  double xs, ys;
+  bool valid;
+};
+
+template <int INTERP>
+__device__ __forceinline__ void burst_adj(const GeomDev &g, const OffPoly &op, const double X[3],
+                                          const double nrm[3], const float2 *__restrict__ z,
+                                          BurstAdj &o) {
+  double t, dr[3], v[3], a[3];
+  zero_doppler(g, X, t, dr, v, a);
+  double rng = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+  double rgoff = (rng - g.rngstart) * g.inv_dmrg;
+  double azoff = (t - g.tstart) * g.inv_dtaz;
+  double xs = rgoff * g.inv_samples, ys = azoff * g.inv_lines;
+  double drg = 0.0, daz = 0.0;
+  if (op.on) {
+    drg = op.p[0] + op.p[1] * xs + op.p[2] * ys;
+    daz = op.p[3] + op.p[4] * xs + op.p[5] * ys;
+  }
+  double rg = rgoff + drg, az = azoff + daz;
+  o.xs = xs;
+  o.ys = ys;
+  if (INTERP == S1IFG_INTERP_BILINEAR)
+    o.valid = rg > g.fvs && rg < g.lvs - 1 && az >= g.flv + 5 && az < g.llv - 5;
+  else
+    o.valid = rg >= g.fvs + 3 && rg < g.lvs - 4 && az >= g.flv + 5 && az < g.llv - 5;
+  if (!o.valid) {
+    o.z = o.dzr = o.dza = make_float2(0.f, 0.f);
+    o.R_rp = o.R_eta = o.dt_dh = o.drng_dh = 0.0;
+    return;
+  }
+  // reramp and its partials (FP64)
+  const double c2 = 2.0 / kSol;
+  double rp = rng + drg * g.dmrg;
+  double tf = rp * c2 - g.fm[0];
+  double ka = g.fm[1] + tf * (g.fm[2] + g.fm[3] * tf);
+  double dka = c2 * (g.fm[2] + 2.0 * g.fm[3] * tf);
+  double td = rp * c2 - g.dc[0];
+  double fnc = g.dc[1] + td * (g.dc[2] + g.dc[3] * td);
+  double dfnc = c2 * (g.dc[2] + 2.0 * g.dc[3] * td);
+  double ikk = rcp64(ka - g.ks), ika = rcp64(ka);
+  double kt = ka * g.ks * ikk;
+  double dkt = -g.ks * g.ks * ikk * ikk * dka;
+  double etaref = -fnc * ika - g.etac0;
+  double detaref = -(dfnc * ka - fnc * dka) * ika * ika;
+  double eta = -g.lines * g.dtaz / 2. + (t - g.tstart) + daz * g.dtaz;
+  double ed = eta - etaref;
+  o.R_rp = -kPi * (dkt * ed * ed - 2.0 * kt * ed * detaref);
+  o.R_eta = -2.0 * kPi * kt * ed;
+  float ph = reduce_phase(4.0 * kPi / g.wvl * rng + kPi * kt * ed * ed);
+  float sp, cp;
+  sincosf(ph, &sp, &cp);
+  // interpolation and its derivatives at the sampling position
+  const int N = g.samples;
+  float vx, vy, rx_, ry_, ax_, ay_;
+  if (INTERP == S1IFG_INTERP_BILINEAR) {
+    int intr = (int)rg, inta = (int)az;
+    float fr = (float)(rg - intr), fa = (float)(az - inta);
+    size_t i1 = (size_t)inta * N + intr;
+    float2 b1 = z[i1], b2 = z[i1 + 1], b3 = z[i1 + N], b4 = z[i1 + N + 1];
+    float c1x = b1.x + (b2.x - b1.x) * fr, c1y = b1.y + (b2.y - b1.y) * fr;
+    float c2x = b3.x + (b4.x - b3.x) * fr, c2y = b3.y + (b4.y - b3.y) * fr;
+    vx = c1x + (c2x - c1x) * fa;
+    vy = c1y + (c2y - c1y) * fa;
+    rx_ = (b2.x - b1.x) * (1.f - fa) + (b4.x - b3.x) * fa;
+    ry_ = (b2.y - b1.y) * (1.f - fa) + (b4.y - b3.y) * fa;
+    ax_ = c2x - c1x;
+    ay_ = c2y - c1y;
+  } else {
+    int intr = (int)floor(rg), inta = (int)floor(az);
+    float wr[8], dwr[8], wa[8], dwa[8];
+    sinc_weights_d((float)(rg - intr), wr, dwr);
+    sinc_weights_d((float)(az - inta), wa, dwa);
+    vx = vy = rx_ = ry_ = ax_ = ay_ = 0.f;
+    const float2 *base = z + (size_t)(inta - 3) * N + (intr - 3);
+    for (int ia = 0; ia < 8; ++ia) {
+      const float2 *row = base + (size_t)ia * N;
+      float sx = 0.f, sy = 0.f, dx = 0.f, dy = 0.f;
+#pragma unroll
+      for (int ir = 0; ir < 8; ++ir) {
+        float2 q = row[ir];
+        sx = fmaf(wr[ir], q.x, sx);
+        sy = fmaf(wr[ir], q.y, sy);
+        dx = fmaf(dwr[ir], q.x, dx);
+        dy = fmaf(dwr[ir], q.y, dy);
+      }
+      vx = fmaf(wa[ia], sx, vx);
+      vy = fmaf(wa[ia], sy, vy);
+      rx_ = fmaf(wa[ia], dx, rx_);
+      ry_ = fmaf(wa[ia], dy, ry_);
+      ax_ = fmaf(dwa[ia], sx, ax_);
+      ay_ = fmaf(dwa[ia], sy, ay_);
+    }
+  }
+  o.z = make_float2(vx * cp - vy * sp, vx * sp + vy * cp);
+  o.dzr = make_float2(rx_ * cp - ry_ * sp, rx_ * sp + ry_ * cp);
+  o.dza = make_float2(ax_ * cp - ay_ * sp, ax_ * sp + ay_ * cp);
+  // height partials
+  double vn = v[0] * nrm[0] + v[1] * nrm[1] + v[2] * nrm[2];
+  double den = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] - (dr[0] * a[0] + dr[1] * a[1] + dr[2] * a[2]);
+  o.dt_dh = vn * rcp64(den);
+  o.drng_dh = (dr[0] * nrm[0] + dr[1] * nrm[1] + dr[2] * nrm[2]) * rcp64(rng);
+}
+
+// Chain from dL/dout of one burst to the offset-polynomial partials (added to
+// gp when given: the secondary) and the height gradient (returned).
+__device__ __forceinline__ double burst_chain(const GeomDev &g, const OffPoly &op,
+                                              const BurstAdj &b, float2 G, double gp[6]) {
+  if (!b.valid) return 0.0;
+  double gRG = (double)G.x * b.dzr.x + (double)G.y * b.dzr.y;
+  double gAZ = (double)G.x * b.dza.x + (double)G.y * b.dza.y;
+  // Re(conj(G) * i * z) = -Im(conj(G) z) = -(G.x z.y - G.y z.x)
+  double gPHI = -((double)G.x * b.z.y - (double)G.y * b.z.x);
+  double gRP = -gPHI * b.R_rp;
+  double gETA = -gPHI * b.R_eta;
+  double gDRG = gRG + gRP * g.dmrg;
+  double gDAZ = gAZ + gETA * g.dtaz;
+  double g_rgoff = gRG, g_azoff = gAZ;
+  if (gp) {
+    gp[0] += gDRG;
+    gp[1] += gDRG * b.xs;
+    gp[2] += gDRG * b.ys;
+    gp[3] += gDAZ;
+    gp[4] += gDAZ * b.xs;
+    gp[5] += gDAZ * b.ys;
+  }
+  if (op.on) {
+    g_rgoff += (gDRG * op.p[1] + gDAZ * op.p[4]) * g.inv_samples;
+    g_azoff += (gDRG * op.p[2] + gDAZ * op.p[5]) * g.inv_lines;
+  }
+  double g_rng = gPHI * (4.0 * kPi / g.wvl) + g_rgoff * g.inv_dmrg + gRP;
+  double g_t = g_azoff * g.inv_dtaz + gETA;
+  return g_t * b.dt_dh + g_rng * b.drng_dh;
+}
+
+constexpr int kAdjBlock = 128;
+
+__global__ void k_fill(float *p, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(kAdjBlock) void k_pair_bwd(
+    const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr,
+    DemDev dem, const RowTab *rows, const ColTab *cols, const float *dh,
+    const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl, int cl, int nr_ml,
+    int nc_ml, const float2 *__restrict__ g_ifg, const float *__restrict__ g_pr,
+    const float *__restrict__ g_ps, double *__restrict__ partial, float *__restrict__ g_dh) {
+  const GeomDev &Gr = *gpr, &Gs = *gps;
+  const OffPoly none = {{0, 0, 0, 0, 0, 0}, 0};
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double gp[6] = {0, 0, 0, 0, 0, 0};
+  if (idx < (long)nr_ml * nc_ml) {
+    const int rm = (int)(idx / nc_ml), cm = (int)(idx - (long)rm * nc_ml);
+    const float inv_n = 1.0f / (float)(rl * cl);
+    float2 gI = g_ifg ? g_ifg[idx] : make_float2(0.f, 0.f);
+    gI.x *= inv_n;
+    gI.y *= inv_n;
+    const float g1 = g_pr ? 2.0f * g_pr[idx] * inv_n : 0.f;
+    const float g2 = g_ps ? 2.0f * g_ps[idx] * inv_n : 0.f;
+    for (int i = 0; i < rl; ++i) {
+      const int r = rm * rl + i;
+      const RowTab rt = rows[r];
+      for (int j = 0; j < cl; ++j) {
+        const int c = cm * cl + j;
+        const ColTab ct = cols[c];
+        double X[3];
+        pixel_xyz(gr, dem, rows, cols, dh, r, c, X);
+        const double nrm[3] = {rt.coslat * ct.coslon, rt.coslat * ct.sinlon, rt.sinlat};
+        BurstAdj br, bs;
+        burst_adj<INTERP>(Gr, none, X, nrm, zr, br);
+        burst_adj<INTERP>(Gs, ops, X, nrm, zs, bs);
+        // G_r = conj(gI) z_s + g1 z_r ; G_s = gI z_r + g2 z_s
+        float2 Gref = make_float2(gI.x * bs.z.x + gI.y * bs.z.y + g1 * br.z.x,
+                                  gI.x * bs.z.y - gI.y * bs.z.x + g1 * br.z.y);
+        float2 Gsec = make_float2(gI.x * br.z.x - gI.y * br.z.y + g2 * bs.z.x,
+                                  gI.x * br.z.y + gI.y * br.z.x + g2 * bs.z.y);
+        double gh = burst_chain(Gr, none, br, Gref, nullptr) + burst_chain(Gs, ops, bs, Gsec, gp);
+        if (g_dh) g_dh[(size_t)r * gr.ncol + c] = (float)gh;
+      }
+    }
+  }
+  // block reduction of the 6 partials (fixed tree order)
+  __shared__ double red[6][kAdjBlock];
+  for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = gp[k];
+  __syncthreads();
+  for (int s = kAdjBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) partial[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Sum of the per-block partials in block order, one thread per coefficient.
+__global__ void k_reduce6(const double *__restrict__ partial, int nblocks, double *out) {
+  const int k = threadIdx.x;
+  if (k >= 6) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * 6 + k];
+  out[k] = s;
+}
+
+// Coherence adjoint, step 1: per output pixel q the coefficients of the
+// window sums (cA complex, cB, cC); win = 1 gives the final gradients.
+__global__ void k_coh_bwd_coef(const float2 *__restrict__ ifg, const float *__restrict__ p1,
+                               const float *__restrict__ p2, int nr, int nc, int win,
+                               const float *__restrict__ g, float2 *__restrict__ cA,
+                               float *__restrict__ cB, float *__restrict__ cC) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (r >= nr || c >= nc) return;
+  size_t o = (size_t)r * nc + c;
+  float gq = g[o];
+  float2 ra = make_float2(0.f, 0.f);
+  float rb = 0.f, rc = 0.f;
+  if (win == 1) {
+    float2 v = ifg[o];
+    float a1 = sqrtf(p1[o]) + 1e-12f, a2 = sqrtf(p2[o]) + 1e-12f;
+    float mag = sqrtf(v.x * v.x + v.y * v.y);
+    float gam = mag / a1 / a2;
+    if (gam <= 1.0f && mag > 0.f && p1[o] > 0.f && p2[o] > 0.f) {
+      float s = gq / (a1 * a2 * mag);
+      ra = make_float2(s * v.x, s * v.y);
+      rb = -gq * gam / a1 * 0.5f / sqrtf(p1[o]);
+      rc = -gq * gam / a2 * 0.5f / sqrtf(p2[o]);
+    }
+  } else {
+    int h = win / 2;
+    int r0 = max(r - h, 0), r1 = min(r + h, nr - 1);
+    int c0 = max(c - h, 0), c1 = min(c + h, nc - 1);
+    float sx = 0.f, sy = 0.f, s1 = 0.f, s2 = 0.f;
+    for (int i = r0; i <= r1; ++i)
+      for (int j = c0; j <= c1; ++j) {
+        size_t b = (size_t)i * nc + j;
+        float2 v = ifg[b];
+        sx += v.x;
+        sy += v.y;
+        s1 += p1[b];
+        s2 += p2[b];
+      }
+    float den = sqrtf(s1 * s2);
+    float mag = sqrtf(sx * sx + sy * sy);
+    if (den > 0.f && mag > 0.f) {
+      float gam = mag / den;
+      if (gam <= 1.0f) {
+        float s = gq / (mag * den);
+        ra = make_float2(s * sx, s * sy);
+        rb = -0.5f * gq * gam / s1;
+        rc = -0.5f * gq * gam / s2;
+      }
+    }
+  }
+  cA[o] = ra;
+  cB[o] = rb;
+  cC[o] = rc;
+}
+
+// Step 2 (win >= 3): transpose of the window sums = the same window sums.
+__global__ void k_coh_bwd_box(const float2 *__restrict__ cA, const float *__restrict__ cB,
+                              const float *__restrict__ cC, int nr, int nc, int win,
+                              float2 *__restrict__ gI, float *__restrict__ g1,
+                              float *__restrict__ g2) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (r >= nr || c >= nc) return;
+  int h = win / 2;
+  int r0 = max(r - h, 0), r1 = min(r + h, nr - 1);
+  int c0 = max(c - h, 0), c1 = min(c + h, nc - 1);
+  float sx = 0.f, sy = 0.f, s1 = 0.f, s2 = 0.f;
+  for (int i = r0; i <= r1; ++i)
+    for (int j = c0; j <= c1; ++j) {
+      size_t b = (size_t)i * nc + j;
+      sx += cA[b].x;
+      sy += cA[b].y;
+      s1 += cB[b];
+      s2 += cC[b];
+    }
+  size_t o = (size_t)r * nc + c;
+  if (gI) gI[o] = make_float2(sx, sy);
+  if (g1) g1[o] = s1;
+  if (g2) g2[o] = s2;
+}
+
+}  // namespace
+
+namespace s1ifg {
+int grid_tables(s1ifg_ctx *ctx, const GridDev &gr, const DemDev &dem, RowTab **rows,
+                ColTab **cols, size_t extra, void **extra_ptr);
+}
+
+extern "C" {
+
+int s1ifg_pair_backward(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
+                        const s1ifg_grid *grid, const s1ifg_dem *dem, const float *dh,
+                        const double *offpoly_sec, const s1ifg_cf32 *ref_deramped,
+                        const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl,
+                        const s1ifg_cf32 *g_ifg, const float *g_pow_ref,
+                        const float *g_pow_sec, double *g_offpoly, float *g_dh) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if ((rc = check_geom(ctx, g_ref)) || (rc = check_geom(ctx, g_sec))) return rc;
+  GridDev gr;
+  DemDev dd;
+  OffPoly op;
+  if ((rc = make_grid(grid, &gr)) || (rc = make_dem(dem, &dd)) ||
+      (rc = make_offpoly(offpoly_sec, &op)))
+    return rc;
+  if (!ref_deramped || !sec_deramped) return set_error(S1IFG_EINVAL, "null burst pointer");
+  if (rl < 1 || cl < 1) return set_error(S1IFG_EINVAL, "looks %d x %d", rl, cl);
+  if (interp != S1IFG_INTERP_BILINEAR && interp != S1IFG_INTERP_SINC)
+    return set_error(S1IFG_EINVAL, "interp %d", interp);
+  int nr = gr.nrow / rl, nc = gr.ncol / cl;
+  if (nr < 1 || nc < 1) return set_error(S1IFG_EINVAL, "grid smaller than one look cell");
+  long nml = (long)nr * nc;
+  unsigned blocks = (unsigned)((nml + kAdjBlock - 1) / kAdjBlock);
+  RowTab *rows;
+  ColTab *cols;
+  void *part = nullptr;
+  if ((rc = grid_tables(ctx, gr, dd, &rows, &cols, sizeof(double) * 6 * (size_t)blocks, &part)))
+    return rc;
+  if (g_dh) {
+    long n = (long)gr.nrow * gr.ncol;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       g_dh, n);
+    S1_LAUNCH_CHECK("k_fill");
+  }
+  if (interp == S1IFG_INTERP_BILINEAR)
+    hipLaunchKernelGGL(k_pair_bwd<S1IFG_INTERP_BILINEAR>, dim3(blocks), dim3(kAdjBlock), 0,
+                       ctx->stream, g_ref->d_dev, g_sec->d_dev, op, gr, dd, rows, cols, dh,
+                       (const float2 *)ref_deramped, (const float2 *)sec_deramped, rl, cl, nr, nc,
+                       (const float2 *)g_ifg, g_pow_ref, g_pow_sec, (double *)part, g_dh);
+  else
+    hipLaunchKernelGGL(k_pair_bwd<S1IFG_INTERP_SINC>, dim3(blocks), dim3(kAdjBlock), 0,
+                       ctx->stream, g_ref->d_dev, g_sec->d_dev, op, gr, dd, rows, cols, dh,
+                       (const float2 *)ref_deramped, (const float2 *)sec_deramped, rl, cl, nr, nc,
+                       (const float2 *)g_ifg, g_pow_ref, g_pow_sec, (double *)part, g_dh);
+  S1_LAUNCH_CHECK("k_pair_bwd");
+  if (g_offpoly) {
+    hipLaunchKernelGGL(k_reduce6, dim3(1), dim3(64), 0, ctx->stream, (const double *)part,
+                       (int)blocks, g_offpoly);
+    S1_LAUNCH_CHECK("k_reduce6");
+  }
+  return S1IFG_OK;
+}
+
+int s1ifg_coherence_backward(s1ifg_ctx *ctx, const s1ifg_cf32 *ifg_ml, const float *pow_ref,
+                             const float *pow_sec, int nrow, int ncol, int win,
+                             const float *g_coh, s1ifg_cf32 *g_ifg, float *g_pow_ref,
+                             float *g_pow_sec) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!ifg_ml || !pow_ref || !pow_sec || !g_coh) return set_error(S1IFG_EINVAL, "null pointer");
+  if (nrow < 1 || ncol < 1) return set_error(S1IFG_EINVAL, "bad size");
+  if (win < 1 || win % 2 == 0) return set_error(S1IFG_EINVAL, "window %d (odd, >= 1)", win);
+  size_t n = (size_t)nrow * ncol;
+  dim3 blk(64, 4), grd((ncol + 63) / 64, (nrow + 3) / 4);
+  if (win == 1) {
+    if (!g_ifg || !g_pow_ref || !g_pow_sec)
+      return set_error(S1IFG_EINVAL, "win = 1 needs all three gradient outputs");
+    hipLaunchKernelGGL(k_coh_bwd_coef, grd, blk, 0, ctx->stream, (const float2 *)ifg_ml, pow_ref,
+                       pow_sec, nrow, ncol, 1, g_coh, (float2 *)g_ifg, g_pow_ref, g_pow_sec);
+    S1_LAUNCH_CHECK("k_coh_bwd_coef");
+    return S1IFG_OK;
+  }
+  void *ws = nullptr;
+  if ((rc = ctx_workspace(ctx, n * 16 + 512, &ws))) return rc;
+  float2 *cA = (float2 *)ws;
+  float *cB = (float *)((char *)ws + ((n * 8 + 255) & ~(size_t)255));
+  float *cC = (float *)((char *)cB + ((n * 4 + 255) & ~(size_t)255));
+  hipLaunchKernelGGL(k_coh_bwd_coef, grd, blk, 0, ctx->stream, (const float2 *)ifg_ml, pow_ref,
+                     pow_sec, nrow, ncol, win, g_coh, cA, cB, cC);
+  S1_LAUNCH_CHECK("k_coh_bwd_coef");
+  hipLaunchKernelGGL(k_coh_bwd_box, grd, blk, 0, ctx->stream, cA, cB, cC, nrow, ncol, win,
+                     (float2 *)g_ifg, g_pow_ref, g_pow_sec);
+  S1_LAUNCH_CHECK("k_coh_bwd_box");
+  return S1IFG_OK;
+}
+
+}  // extern "C"

@@ -140,20 +140,36 @@ __global__ void k_rows_final(int *fl, int nrow) {
   }
 }
 
+}  // namespace
+
+namespace s1ifg {
+
+// Grid tables in the context workspace, followed by `extra` bytes of
+// scratch for the caller (256-byte aligned; *extra_ptr may be NULL).
 int grid_tables(s1ifg_ctx *ctx, const GridDev &gr, const DemDev &dem, RowTab **rows,
-                ColTab **cols) {
-  size_t nb = sizeof(RowTab) * (size_t)gr.nrow + sizeof(ColTab) * (size_t)gr.ncol + 256;
+                ColTab **cols, size_t extra, void **extra_ptr) {
+  size_t off_c = (sizeof(RowTab) * (size_t)gr.nrow + 255) & ~(size_t)255;
+  size_t off_x = (off_c + sizeof(ColTab) * (size_t)gr.ncol + 255) & ~(size_t)255;
   void *ws = nullptr;
-  int rc = ctx_workspace(ctx, nb, &ws);
+  int rc = ctx_workspace(ctx, off_x + extra + 256, &ws);
   if (rc) return rc;
   *rows = (RowTab *)ws;
-  size_t off = (sizeof(RowTab) * (size_t)gr.nrow + 255) & ~(size_t)255;
-  *cols = (ColTab *)((char *)ws + off);
+  *cols = (ColTab *)((char *)ws + off_c);
+  if (extra_ptr) *extra_ptr = (char *)ws + off_x;
   int n = gr.nrow > gr.ncol ? gr.nrow : gr.ncol;
   hipLaunchKernelGGL(k_grid_tables, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, gr, dem,
                      *rows, *cols);
   S1_LAUNCH_CHECK("k_grid_tables");
   return S1IFG_OK;
+}
+
+}  // namespace s1ifg
+
+namespace {
+
+int grid_tables(s1ifg_ctx *ctx, const GridDev &gr, const DemDev &dem, RowTab **rows,
+                ColTab **cols) {
+  return s1ifg::grid_tables(ctx, gr, dem, rows, cols, 0, nullptr);
 }
 
 int common_setup(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_grid *grid,

@@ -172,6 +172,47 @@ def pair(ctx, g_ref, g_sec, grid, dem_t, dem, ref_deramped, sec_deramped, interp
     return ifg, pr, ps
 
 
+def pair_backward(ctx, g_ref, g_sec, grid, dem_t, dem, ref_deramped, sec_deramped, g_ifg,
+                  g_pr=None, g_ps=None, interp="bilinear", rl=1, cl=4, dh=None, offpoly=None,
+                  want_dh=True):
+    """Adjoint of pair(): (dL/d offpoly as float64[6] device tensor,
+    dL/d dh as float32 target-grid tensor or None)."""
+    g = N.make_grid(grid)
+    d = _dem_args(dem_t, dem)
+    shape = (grid["nrow"] // rl, grid["ncol"] // cl)
+    for t, name, dt in ((g_ifg, "g_ifg", torch.complex64), (g_pr, "g_pr", torch.float32),
+                        (g_ps, "g_ps", torch.float32)):
+        if t is not None:
+            _need(t, dt, shape, name)
+    if dh is not None:
+        _need(dh, torch.float32, (grid["nrow"], grid["ncol"]), "dh")
+    dev = ref_deramped.device
+    g_off = _new((6,), torch.float64, dev)
+    g_dh = _new((grid["nrow"], grid["ncol"]), torch.float32, dev) if want_dh else None
+    _keep, pp = N.offpoly_arg(offpoly)
+    N.check(ctx.lib.s1ifg_pair_backward(
+        ctx.handle, g_ref.handle, g_sec.handle, ctypes.byref(g), ctypes.byref(d), N.ptr(dh), pp,
+        N.ptr(ref_deramped), N.ptr(sec_deramped), N.INTERP[interp], rl, cl, N.ptr(g_ifg),
+        N.ptr(g_pr), N.ptr(g_ps), N.ptr(g_off), N.ptr(g_dh)))
+    return g_off, g_dh
+
+
+def coherence_backward(ctx, ifg_ml, pow_ref, pow_sec, g_coh, win=1):
+    """Adjoint of coherence(): (g_ifg complex64, g_pow_ref, g_pow_sec)."""
+    _need(ifg_ml, torch.complex64, None, "ifg")
+    shape = tuple(ifg_ml.shape)
+    _need(pow_ref, torch.float32, shape, "pow_ref")
+    _need(pow_sec, torch.float32, shape, "pow_sec")
+    _need(g_coh, torch.float32, shape, "g_coh")
+    gi = _new(shape, torch.complex64, ifg_ml.device)
+    g1 = _new(shape, torch.float32, ifg_ml.device)
+    g2 = _new(shape, torch.float32, ifg_ml.device)
+    N.check(ctx.lib.s1ifg_coherence_backward(ctx.handle, N.ptr(ifg_ml), N.ptr(pow_ref),
+                                             N.ptr(pow_sec), shape[0], shape[1], win,
+                                             N.ptr(g_coh), N.ptr(gi), N.ptr(g1), N.ptr(g2)))
+    return gi, g1, g2
+
+
 def orbit_eval(ctx, geom, t):
     """Orbit position/velocity/acceleration at device times t (float64)."""
     _need(t, torch.float64, None, "t")
