@@ -46,6 +46,9 @@ def parse():
     p.add_argument("--forms", default="plain,fused,batched")
     p.add_argument("--no-iw", action="store_true", help="skip the 27-burst IW pair")
     p.add_argument("--iw-steps", type=int, default=5)
+    p.add_argument("--no-adjoint", action="store_true", help="skip the C2 forward+backward")
+    p.add_argument("--no-stack", action="store_true", help="skip the 20-date stack")
+    p.add_argument("--stack-dates", type=int, default=20)
     p.add_argument("--gold-mode", default="auto", choices=["auto", "fp32", "fp16_split"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", default="c1")
@@ -177,7 +180,182 @@ def run_gpu(args):
     results["_shape"] = (L, Nr, dem["data"].size, rl, cl)
     if "batch" in holder:
         holder["batch"].close()
+    if not args.no_adjoint:
+        try:
+            results["_adjoint"] = run_adjoint(args, ctx, gr, gs, grid, dem_t, dem, dref, dsec,
+                                              offpoly, rl, cl)
+        except Exception as e:  # reported; the forward numbers stand alone
+            results["_adjoint"] = dict(error=repr(e))
     return results
+
+
+def interior_mask(K, ctx, gr, gs, grid, dem_t, dem, dref, dsec, interp, offpoly, dh, rl, cl,
+                  win, margin=2):
+    """Multilooked pixels whose coherence window only sees target pixels
+    valid in both bursts with `margin` valid pixels around them (device
+    bool tensor): finite differences there cannot cross a validity edge."""
+    import torch
+    import torch.nn.functional as F
+    a = K.geocode(ctx, gr, grid, dem_t, dem, dref, interp, dh=dh)
+    b = K.geocode(ctx, gs, grid, dem_t, dem, dsec, interp, dh=dh, offpoly=offpoly)
+    bad = ((a == 0) | (b == 0)).float()[None, None]
+    del a, b
+    bad = F.max_pool2d(bad, 2 * margin + 1, stride=1, padding=margin)[0, 0] > 0
+    nr, nc = grid["nrow"] // rl, grid["ncol"] // cl
+    cell = ~bad[:nr * rl, :nc * cl].reshape(nr, rl, nc, cl).any(dim=3).any(dim=1)
+    h = win // 2
+    badc = F.max_pool2d((~cell).float()[None, None], 2 * h + 1, stride=1, padding=h)[0, 0] > 0
+    cell = ~badc
+    cell[:h, :] = False
+    cell[-h:, :] = False
+    cell[:, :h] = False
+    cell[:, -h:] = False
+    return cell
+
+
+def run_adjoint(args, ctx, gr, gs, grid, dem_t, dem, dref, dsec, offpoly, rl, cl):
+    """C4: loss = -mean(5x5 coherence) over the interior of the C2 pair,
+    gradient with respect to the 6 offset-polynomial coefficients and a
+    per-pixel height error dh ~ N(0, 5 m); forward and backward timed, the
+    gradient checked against central differences of the forward (1e-3 px,
+    1e-2 m along one random direction)."""
+    import numpy as np
+    import torch
+
+    from sentinel_gpu_processor_amd import kernels as K
+
+    dev = dref.device
+    g = torch.Generator(device=dev)
+    g.manual_seed(4)
+    dh = (torch.randn(grid["nrow"], grid["ncol"], device=dev, generator=g) * 5.0).contiguous()
+    p = np.asarray(offpoly, np.float64)
+    interp, win = args.interp, 5
+    mask = interior_mask(K, ctx, gr, gs, grid, dem_t, dem, dref, dsec, interp, p, dh, rl, cl,
+                         win)
+    cnt = int(mask.sum())
+
+    def forward(pp, hh):
+        ifg, pr, ps = K.pair(ctx, gr, gs, grid, dem_t, dem, dref, dsec, interp, rl, cl, dh=hh,
+                             offpoly=pp)
+        coh = K.coherence(ctx, ifg, pr, ps, win=win)
+        return ifg, pr, ps, coh
+
+    def loss(pp, hh):
+        coh = forward(pp, hh)[3]
+        return -float(coh.double()[mask].sum()) / cnt
+
+    gcoh = torch.where(mask, torch.tensor(-1.0 / cnt, device=dev), torch.tensor(0.0, device=dev))
+    gcoh = gcoh.float().contiguous()
+
+    def backward(ifg, pr, ps):
+        gi, g1, g2 = K.coherence_backward(ctx, ifg, pr, ps, gcoh, win=win)
+        return K.pair_backward(ctx, gr, gs, grid, dem_t, dem, dref, dsec, gi, g1, g2, interp, rl,
+                               cl, dh=dh, offpoly=p)
+
+    ifg, pr, ps, coh = forward(p, dh)
+    backward(ifg, pr, ps)
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    reps = 3
+    e[0].record()
+    for _ in range(reps):
+        ifg, pr, ps, coh = forward(p, dh)
+    e[1].record()
+    for _ in range(reps):
+        g_off, g_dh = backward(ifg, pr, ps)
+    e[2].record()
+    torch.cuda.synchronize()
+    ms_f = e[0].elapsed_time(e[1]) / reps
+    ms_b = e[1].elapsed_time(e[2]) / reps
+    gp = g_off.cpu().numpy()
+    fd = []
+    for k in range(6):
+        d = np.zeros(6)
+        d[k] = 1e-3
+        fd.append((loss(p + d, dh) - loss(p - d, dh)) / 2e-3)
+    fd = np.array(fd)
+    gd = torch.Generator(device=dev)
+    gd.manual_seed(5)
+    delta = torch.randn(grid["nrow"], grid["ncol"], device=dev, generator=gd)
+    fd_h = (loss(p, dh + 1e-2 * delta) - loss(p, dh - 1e-2 * delta)) / 2e-2
+    an_h = float((g_dh.double() * delta.double()).sum())
+    rel = np.abs(gp - fd) / np.maximum(np.abs(fd), 1e-12)
+    res = dict(workload=f"c4: c2 pair, loss = -mean(5x5 coherence) over {cnt} interior "
+                        f"multilooked pixels, grads wrt offset polynomial (6) and dh "
+                        f"({grid['nrow']}x{grid['ncol']}, N(0, 5 m))",
+               forward_ms=round(ms_f, 3), backward_ms=round(ms_b, 3),
+               grad_offpoly=[float(v) for v in gp], fd_offpoly=[float(v) for v in fd],
+               fd_rel_err_offpoly=[float(v) for v in rel],
+               dh_directional=dict(analytic=an_h, fd=fd_h,
+                                   rel_err=abs(an_h - fd_h) / max(abs(fd_h), 1e-30)))
+    print(f"[bench] adjoint: forward {ms_f:.3f} ms backward {ms_b:.3f} ms; offpoly fd rel "
+          f"err max {rel.max():.2e}; dh directional rel err {res['dh_directional']['rel_err']:.2e}",
+          file=sys.stderr, flush=True)
+    return res
+
+
+def run_stack(args, gold_mode):
+    """C5: 20-date stack (19 sequential pairs) of 1500 x 25000 bursts with
+    per-date atmospheric phase screens, one batched graph vs a per-pair
+    fused loop (each date deramped once in the batch)."""
+    import torch
+
+    from sentinel_gpu_processor_amd import _native as N
+    from sentinel_gpu_processor_amd import autotune
+    from sentinel_gpu_processor_amd import kernels as K
+    from sentinel_gpu_processor_amd import synthetic as S
+    from sentinel_gpu_processor_amd.stack import Stack
+
+    dev = torch.device("cuda:0")
+    cfg = S.CONFIGS["c2"]
+    t0 = time.perf_counter()
+    st = S.make_stack(n_dates=args.stack_dates, lines=cfg["lines"], samples=cfg["samples"],
+                      dem_shape=cfg["dem"], dem_kind=cfg["dem_kind"], hmax=cfg["hmax"], seed=0,
+                      fidelity="fast", device=dev)
+    print(f"[bench] stack: generated {len(st['dates'])} dates in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    ctx = N.Context(0)
+    dem_t = torch.from_numpy(st["dem"]["data"]).to(dev)
+    run = Stack(ctx, st["dates"], st["grid"], dem_t, st["dem"], interp=args.interp, rl=1, cl=4,
+                gold_mode=gold_mode, coh_win=5)
+    shape, _, _ = autotune.tune(run.batch, reps=1)
+    prods = run.products()
+    grid = st["grid"]
+
+    def batched(ev):
+        ev.mark("stack")
+        run.run(graph=True)
+        ev.mark(None)
+
+    def fused(ev):
+        ev.mark("loop")
+        for k, (a, b) in enumerate(run.pairs):
+            K.deramp(ctx, run.geoms[a], run.raw[a], out=run.deramped[a])
+            K.deramp(ctx, run.geoms[b], run.raw[b], out=run.deramped[b])
+            o = prods[k]
+            K.pair(ctx, run.geoms[a], run.geoms[b], grid, dem_t, st["dem"], run.deramped[a],
+                   run.deramped[b], args.interp, 1, 4, out=(o["ifg"], o["pow_ref"], o["pow_sec"]))
+            K.goldstein(ctx, o["ifg"], 0.5, mode=gold_mode, out=o["filt"])
+            K.coherence(ctx, o["ifg"], o["pow_ref"], o["pow_sec"], win=5, out=o["coh"])
+        ev.mark(None)
+
+    npx = len(run.pairs) * grid["nrow"] * grid["ncol"]
+    out = {}
+    for name, fn in (("fused_loop", fused), ("batched", batched)):
+        r = time_form(fn, 3, 1, "stack " + name)
+        out[name] = dict(ms_per_step=round(r["ms_per_step"], 3),
+                         mpix_s=round(npx / (r["ms_per_step"] * 1e-3) / 1e6, 1))
+    res = dict(workload=f"c5: {len(st['dates'])}-date stack, {len(run.pairs)} sequential pairs of "
+                        f"{grid['nrow']}x{grid['ncol']}, APS 2 km / 1 rad, {args.interp}, "
+                        f"{gold_mode}",
+               target_px=npx, launch=shape, forms=out,
+               batched_vs_fused_loop=round(out["fused_loop"]["ms_per_step"]
+                                           / out["batched"]["ms_per_step"], 3),
+               scratch_bytes=run.batch.scratch_bytes(),
+               errors="per-pair RMS phase / coherence / Goldstein errors against the FP64 "
+                      "oracle: tests/test_gpu_stack.py (20 dates, 128 x 1024 crops)")
+    run.close()
+    return res
 
 
 def time_form(fn, steps, warmup, name):
@@ -432,6 +610,12 @@ def main():
             iw = run_iw(args, res["_gold_mode"])
         except Exception as e:  # reported; the one-pair number stands alone
             iw = dict(error=repr(e))
+    stack = None
+    if not args.no_stack:
+        try:
+            stack = run_stack(args, res["_gold_mode"])
+        except Exception as e:  # reported; the one-pair number stands alone
+            stack = dict(error=repr(e))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -460,6 +644,8 @@ def main():
         "forms": forms,
         "launch": res.get("_launch"),
         "iw": iw,
+        "adjoint": res.get("_adjoint"),
+        "stack": stack,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -268,9 +268,12 @@ int s1ifg_coherence_backward(s1ifg_ctx *ctx, const s1ifg_cf32 *ifg_ml,
 /* ------------------------------------------------------------ batches */
 /* One burst pair of a batch (all pointers DEVICE except the geometries).
  * The raw bursts are deramped into ref_deramped / sec_deramped, which must
- * not alias the raw bursts (a batch may be run many times).  The products
- * are on the multilooked grid (grid.nrow/rl x grid.ncol/cl); filt and coh
- * may be NULL to skip the Goldstein filter / coherence of that pair. */
+ * not alias the raw bursts (a batch may be run many times).  A NULL raw
+ * burst means its deramped buffer is an input.  Pairs may share bursts (a
+ * stack of sequential pairs): a burst given by the same raw / deramped
+ * buffers in several tasks is deramped once.  The products are on the
+ * multilooked grid (grid.nrow/rl x grid.ncol/cl); filt and coh may be NULL
+ * to skip the Goldstein filter / coherence of that pair. */
 typedef struct {
   const s1ifg_geom *g_ref, *g_sec;
   s1ifg_grid grid;

@@ -24,7 +24,8 @@ class PairBatch:
       g_ref, g_sec      Geometry of the reference / secondary burst
       grid              target grid dict (nrow, ncol, lat0, lon0, dlat, dlon)
       dem_t, dem        DEM tensor (float32 or int16) and its grid dict
-      ref_slc, sec_slc  raw bursts, complex64 (lines x samples)
+      ref_slc, sec_slc  raw bursts, complex64 (lines x samples); None when
+                        ref_deramped / sec_deramped hold the deramped burst
       dh, offpoly       optional height error (float32, grid shape) and the
                         secondary's 6-coefficient offset polynomial
     Output buffers (ref_deramped, sec_deramped, ifg, pow_ref, pow_sec, filt,
@@ -44,17 +45,22 @@ class PairBatch:
         tasks = (N.PairTask * n)()
         fill = {"clamp": N.COH_FILL_CLAMP, "nan": N.COH_FILL_NAN, "hash": N.COH_FILL_HASH}[coh_fill]
         for k, p in enumerate(pairs):
-            dev = p["ref_slc"].device
             br, bs = p["g_ref"].burst, p["g_sec"].burst
-            for name, t, shape in (("ref_slc", p["ref_slc"], (br["lines"], br["samples"])),
-                                   ("sec_slc", p["sec_slc"], (bs["lines"], bs["samples"]))):
-                if t.dtype != torch.complex64 or tuple(t.shape) != shape:
-                    raise ValueError(f"pair {k}: {name} must be complex64 {shape}")
+            q = dict(p)
+            for side, b in (("ref", br), ("sec", bs)):
+                shape = (b["lines"], b["samples"])
+                raw = q.get(f"{side}_slc")
+                if q.get(f"{side}_deramped") is None:
+                    if raw is None:
+                        raise ValueError(f"pair {k}: {side}_slc or {side}_deramped is needed")
+                    q[f"{side}_deramped"] = torch.empty_like(raw)
+                for name in (f"{side}_slc", f"{side}_deramped"):
+                    t = q.get(name)
+                    if t is not None and (t.dtype != torch.complex64 or tuple(t.shape) != shape):
+                        raise ValueError(f"pair {k}: {name} must be complex64 {shape}")
+            dev = q["ref_deramped"].device
             g = p["grid"]
             nr, nc = g["nrow"] // rl, g["ncol"] // cl
-            q = dict(p)
-            q.setdefault("ref_deramped", torch.empty_like(p["ref_slc"]))
-            q.setdefault("sec_deramped", torch.empty_like(p["sec_slc"]))
             if q.get("ifg") is None:
                 q["ifg"] = torch.empty((nr, nc), dtype=torch.complex64, device=dev)
             if q.get("pow_ref") is None:
@@ -74,8 +80,8 @@ class PairBatch:
             if q.get("offpoly") is not None:
                 t.offpoly_sec[:] = [float(v) for v in np.asarray(q["offpoly"], np.float64).reshape(6)]
                 t.use_offpoly = 1
-            t.ref_slc = _ptrval(q["ref_slc"])
-            t.sec_slc = _ptrval(q["sec_slc"])
+            t.ref_slc = _ptrval(q.get("ref_slc"))
+            t.sec_slc = _ptrval(q.get("sec_slc"))
             t.ref_deramped = _ptrval(q["ref_deramped"])
             t.sec_deramped = _ptrval(q["sec_deramped"])
             t.ifg_ml = _ptrval(q["ifg"])

@@ -33,8 +33,7 @@ struct PairDev {
   GridDev grid;
   DemDev dem;
   const float *dh;
-  const float2 *ref_in, *sec_in;
-  float2 *ref_d, *sec_d;
+  const float2 *ref_d, *sec_d;
   RowTab *rows;
   ColTab *cols;
   float2 *ifg;
@@ -42,6 +41,13 @@ struct PairDev {
   float2 *patches, *filt;
   float *coh;
   int nr_ml, nc_ml, pi_n, pj_n;
+};
+
+// One burst to deramp (bursts shared by several pairs of a stack appear once).
+struct ImgDev {
+  const S1C GeomDev *g;
+  const float2 *in;
+  float2 *out;
 };
 
 enum { kStTables, kStDeramp, kStPair, kStGold, kStGather, kStCoh, kNumStages };
@@ -72,17 +78,17 @@ __global__ void kb_tables(const S1C PairDev *__restrict__ P, const S1C long *__r
   }
 }
 
-// Deramp of every burst (image 2p = reference of pair p, 2p + 1 = secondary);
-// one block per burst line, the element code of k_deramp.
-__global__ __launch_bounds__(256) void kb_deramp(const S1C PairDev *__restrict__ P,
+// Deramp of every distinct burst of the batch; one block per burst line,
+// the element code of k_deramp.
+__global__ __launch_bounds__(256) void kb_deramp(const S1C ImgDev *__restrict__ I,
                                                  const S1C long *__restrict__ off, int nimg) {
   const long total = off[nimg];
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
     const int k = find_item(off, nimg, t);
-    const PairDev &d = *(const PairDev *)&P[k >> 1];
-    const GeomDev &g = geo((k & 1) ? d.gs : d.gr);
-    const float2 *in = (k & 1) ? d.sec_in : d.ref_in;
-    float2 *out = (k & 1) ? d.sec_d : d.ref_d;
+    const ImgDev &im = *(const ImgDev *)&I[k];
+    const GeomDev &g = geo(im.g);
+    const float2 *in = im.in;
+    float2 *out = im.out;
     const int i = (int)(t - off[k]);
     const int N = g.samples;
     const double eta = -g.lines * g.ati / 2. + i * g.ati;
@@ -240,9 +246,11 @@ struct s1ifg_batch {
   s1ifg_batch_opts opts;
   std::vector<long> off[kNumStages];  // host copies of the prefix arrays
   std::vector<PairDev> pairs;
+  std::vector<ImgDev> imgs;
   void *d_buf = nullptr;
   size_t bytes = 0;
   PairDev *d_pairs = nullptr;
+  ImgDev *d_imgs = nullptr;
   long *d_off[kNumStages] = {};
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -306,7 +314,7 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
   }
   if (long t = total(kStDeramp)) {
     hipLaunchKernelGGL(kb_deramp, dim3(grid_of(t, 1, ncu * o.stream_blocks_per_cu)), dim3(256), 0,
-                       s, pairs, off[kStDeramp], 2 * n);
+                       s, (const S1C ImgDev *)b->d_imgs, off[kStDeramp], (int)b->imgs.size());
     S1_LAUNCH_CHECK("kb_deramp");
   }
   if (long t = total(kStPair)) {
@@ -376,6 +384,8 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
     return set_error(S1IFG_EINVAL, "coherence fill %d", o.coh_fill);
 
   std::vector<PairDev> pairs(n);
+  std::vector<ImgDev> imgs;
+  std::vector<const s1ifg_geom *> img_geom;
   size_t tab_bytes = 0, patch_bytes = 0;
   std::vector<size_t> tab_off(n), patch_off(n);
   for (int p = 0; p < n; ++p) {
@@ -385,16 +395,38 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
     if ((rc = check_geom(ctx, t.g_ref)) || (rc = check_geom(ctx, t.g_sec))) return rc;
     if ((rc = make_grid(&t.grid, &d.grid)) || (rc = make_dem(&t.dem, &d.dem))) return rc;
     if ((rc = make_offpoly(t.use_offpoly ? t.offpoly_sec : nullptr, &d.op))) return rc;
-    if (!t.ref_slc || !t.sec_slc || !t.ref_deramped || !t.sec_deramped || !t.ifg_ml)
+    if (!t.ref_deramped || !t.sec_deramped || !t.ifg_ml)
       return set_error(S1IFG_EINVAL, "pair %d: null data pointer", p);
     if (t.ref_deramped == t.ref_slc || t.sec_deramped == t.sec_slc ||
         t.ref_deramped == t.sec_deramped)
       return set_error(S1IFG_EINVAL, "pair %d: deramped bursts must not alias the raw bursts", p);
+    // distinct bursts to deramp (raw NULL: the deramped buffer is an input)
+    for (int side = 0; side < 2; ++side) {
+      const s1ifg_geom *g = side ? t.g_sec : t.g_ref;
+      const void *raw = side ? (const void *)t.sec_slc : (const void *)t.ref_slc;
+      void *der = side ? (void *)t.sec_deramped : (void *)t.ref_deramped;
+      bool seen = false;
+      for (size_t q = 0; q < imgs.size(); ++q) {
+        if (imgs[q].out == (float2 *)der || (raw && imgs[q].in == (const float2 *)raw)) {
+          if (imgs[q].out != (float2 *)der || imgs[q].in != (const float2 *)raw ||
+              img_geom[q] != g)
+            return set_error(S1IFG_EINVAL,
+                             "pair %d: a burst buffer is shared with a different burst", p);
+          seen = true;
+        }
+      }
+      if (!seen && raw) {
+        ImgDev im;
+        im.g = (const S1C GeomDev *)g->d_dev;
+        im.in = (const float2 *)raw;
+        im.out = (float2 *)der;
+        imgs.push_back(im);
+        img_geom.push_back(g);
+      }
+    }
     d.gr = (const S1C GeomDev *)t.g_ref->d_dev;
     d.gs = (const S1C GeomDev *)t.g_sec->d_dev;
     d.dh = t.dh;
-    d.ref_in = (const float2 *)t.ref_slc;
-    d.sec_in = (const float2 *)t.sec_slc;
     d.ref_d = (float2 *)t.ref_deramped;
     d.sec_d = (float2 *)t.sec_deramped;
     d.ifg = (float2 *)t.ifg_ml;
@@ -427,15 +459,16 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
   b->n = n;
   b->opts = o;
   b->pairs = pairs;
+  b->imgs = imgs;
+  const int nimg = (int)imgs.size();
   // prefix arrays
-  for (int st = 0; st < kNumStages; ++st) b->off[st].assign(st == kStDeramp ? 2 * n + 1 : n + 1, 0);
+  for (int st = 0; st < kNumStages; ++st) b->off[st].assign(st == kStDeramp ? nimg + 1 : n + 1, 0);
+  for (int q = 0; q < nimg; ++q)
+    b->off[kStDeramp][q + 1] = b->off[kStDeramp][q] + img_geom[q]->dev.lines;
   for (int p = 0; p < n; ++p) {
     const PairDev &d = b->pairs[p];
-    const s1ifg_pair_task &t = tasks[p];
     long tab = d.grid.nrow > d.grid.ncol ? d.grid.nrow : d.grid.ncol;
     b->off[kStTables][p + 1] = b->off[kStTables][p] + tab;
-    b->off[kStDeramp][2 * p + 1] = b->off[kStDeramp][2 * p] + t.g_ref->dev.lines;
-    b->off[kStDeramp][2 * p + 2] = b->off[kStDeramp][2 * p + 1] + t.g_sec->dev.lines;
     b->off[kStGold][p + 1] = b->off[kStGold][p] + (d.filt ? (long)d.pi_n * d.pj_n : 0);
     long gtiles = (long)((d.nc_ml + 63) / 64) * ((d.nr_ml + 3) / 4);
     b->off[kStGather][p + 1] = b->off[kStGather][p] + (d.filt ? gtiles : 0);
@@ -447,6 +480,8 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
   // one device buffer: descriptors | prefix arrays | grid tables | patches
   size_t off_pairs = 0;
   size_t sz = align256(sizeof(PairDev) * n);
+  size_t off_imgs = sz;
+  sz += align256(sizeof(ImgDev) * (nimg > 0 ? nimg : 1));
   size_t off_pref[kNumStages];
   for (int st = 0; st < kNumStages; ++st) {
     off_pref[st] = sz;
@@ -465,6 +500,7 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
   b->bytes = sz;
   char *base = (char *)b->d_buf;
   b->d_pairs = (PairDev *)(base + off_pairs);
+  b->d_imgs = (ImgDev *)(base + off_imgs);
   for (int st = 0; st < kNumStages; ++st) b->d_off[st] = (long *)(base + off_pref[st]);
   for (int p = 0; p < n; ++p) {
     PairDev &d = b->pairs[p];
@@ -473,7 +509,9 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
     d.patches = d.filt ? (float2 *)(base + off_patch + patch_off[p]) : nullptr;
   }
   int err = S1IFG_OK;
-  if (hipMemcpy(b->d_pairs, b->pairs.data(), sizeof(PairDev) * n, hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpy(b->d_pairs, b->pairs.data(), sizeof(PairDev) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      (nimg > 0 && hipMemcpy(b->d_imgs, b->imgs.data(), sizeof(ImgDev) * nimg,
+                             hipMemcpyHostToDevice) != hipSuccess))
     err = set_error(S1IFG_EHIP, "descriptor upload failed");
   for (int st = 0; st < kNumStages && !err; ++st) err = upload_prefix(b, st);
   if (!err && (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||

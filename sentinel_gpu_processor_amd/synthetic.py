@@ -294,18 +294,93 @@ def make_iw(seed=0, n_sub=3, n_burst=9, lines=(1450, 1530), samples=(20000, 2550
     return out
 
 
+GROUND_RANGE_PIXEL = 4.0        # m, ground spacing of a range sample (approx.)
+GROUND_AZIMUTH_PIXEL = 13.9     # m, ground spacing of an azimuth line (approx.)
+
+
+def aps_field(lines, samples, seed, corr=2000.0, std=1.0, coarse=16):
+    """Atmospheric phase screen (radians) in radar geometry: a Gaussian random
+    field with Gaussian covariance exp(-r^2 / corr^2) (e-folding distance
+    `corr` metres on the ground) and standard deviation `std`, built on a grid
+    `coarse` pixels apart and upsampled bilinearly (numpy, float32)."""
+    rng = np.random.default_rng(seed)
+    cy = lines // coarse + 3
+    cx = samples // coarse + 3
+    w = rng.standard_normal((cy, cx))
+    # white noise smoothed by a Gaussian of std corr / 2 has covariance
+    # exp(-r^2 / corr^2)
+    sy = corr / 2 / (GROUND_AZIMUTH_PIXEL * coarse)
+    sx = corr / 2 / (GROUND_RANGE_PIXEL * coarse)
+    ky = np.fft.fftfreq(cy)[:, None]
+    kx = np.fft.fftfreq(cx)[None, :]
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * np.exp(-2 * np.pi ** 2 * ((ky * sy) ** 2 + (kx * sx) ** 2))))
+    f = (f - f.mean()) / f.std() * std
+    yi = np.arange(lines) / coarse
+    xi = np.arange(samples) / coarse
+    iy = np.floor(yi).astype(np.int64)
+    ix = np.floor(xi).astype(np.int64)
+    wy = (yi - iy)[:, None]
+    wx = (xi - ix)[None, :]
+    out = ((f[iy][:, ix] * (1 - wx) + f[iy][:, ix + 1] * wx) * (1 - wy)
+           + (f[iy + 1][:, ix] * (1 - wx) + f[iy + 1][:, ix + 1] * wx) * wy)
+    return out.astype(np.float32)
+
+
+def make_stack(n_dates=20, lines=1500, samples=25000, dem_shape=(2000, 3000),
+               dem_kind="fractal", hmax=2500.0, seed=0, gamma0=0.8, bperp_max=150.0,
+               aps_corr=2000.0, aps_std=1.0, fidelity="fast", device=None):
+    """Synthetic stack of n_dates acquisitions of one burst footprint.
+
+    Date 0 is the reference geometry; date k > 0 flies a parallel orbit with
+    a perpendicular baseline drawn uniformly from [-bperp_max, bperp_max]
+    (|b| >= 10 m) and carries the common speckle (coherence gamma0 with date
+    0), its own noise, and its own atmospheric phase screen (aps_field,
+    corr metres, aps_std radians); date 0 carries a screen too.  Returns a
+    dict with the per-date orbits, bursts and SLCs, the shared grid and DEM,
+    the baselines and the screens."""
+    rng = np.random.default_rng(seed + 7919)
+    b = rng.uniform(-bperp_max, bperp_max, n_dates)
+    b = np.where(np.abs(b) < 10.0, np.copysign(10.0, b), b)
+    b[0] = 0.0
+    screens = [aps_field(lines, samples, seed * 1000 + 500 + k, aps_corr, aps_std)
+               for k in range(n_dates)]
+    dates = []
+    base = None
+    for k in range(1, n_dates):
+        sc = make_scene(lines=lines, samples=samples, dem_shape=dem_shape, dem_kind=dem_kind,
+                        hmax=hmax, seed=seed, gamma0=gamma0, baseline=(float(b[k]), 0.0),
+                        amb_height=500.0 * 120.0 / float(b[k]), fidelity=fidelity,
+                        device=device, sec_seed=seed * 1000 + k, aps_sec=screens[k])
+        if base is None:
+            base = sc
+            ref = sc["ref_slc"]
+            if fidelity == "exact":
+                ref = (ref.astype(np.complex128) * np.exp(1j * screens[0])).astype(np.complex64)
+            else:
+                import torch
+                a0 = torch.as_tensor(screens[0], device=ref.device)
+                ref = (ref * torch.polar(torch.ones_like(a0), a0)).to(torch.complex64)
+            dates.append(dict(orbit=sc["orbit_ref"], burst=sc["burst_ref"], slc=ref))
+        dates.append(dict(orbit=sc["orbit_sec"], burst=sc["burst_sec"], slc=sc["sec_slc"]))
+        sc["ref_slc"] = None
+    return dict(dates=dates, grid=base["grid"], dem=base["dem"], bperp=b, aps=screens)
+
+
 def make_scene(lines=512, samples=2048, dem_shape=(256, 256), dem_kind="hill", hmax=1500.0,
                seed=0, gamma0=0.8, phase_noise=0.5, amb_height=500.0, heading=-10.0,
                look_range=800.0e3, baseline=(120.0, 30.0), misreg=None, misreg_ramp=(0.0, 0.0),
                fidelity="exact", device=None, lat_c=34.0, lon_c=-118.0, t_center=43200.0,
-               margin=24):
+               margin=24, sec_seed=None, aps_sec=None):
     """Build one synthetic burst pair.
 
     misreg: (range px, azimuth px) constant misregistration of the secondary
     content (corrected by the offset polynomial p = [drg, rg_ramp, 0, daz,
     0, az_ramp]); misreg_ramp: linear ramps (pixels across the burst in range
-    for both).  Returns a dict with orbits, bursts, grid, dem, SLCs and the
-    offset polynomial that realigns the secondary.
+    for both).  sec_seed: separate seed of the secondary's own noise (the
+    reference speckle still follows `seed`); aps_sec: phase screen (radians,
+    lines x samples) multiplied into the secondary.  Returns a dict with
+    orbits, bursts, grid, dem, SLCs and the offset polynomial that realigns
+    the secondary.
     """
     rng = np.random.default_rng(seed)
     L, N = lines, samples
@@ -381,11 +456,14 @@ def make_scene(lines=512, samples=2048, dem_shape=(256, 256), dem_kind="hill", h
                line_orbits=(orb_ref, orb_sec))
     if fidelity == "exact":
         ref, sec = _slcs_exact(rng, orb_ref, orb_sec, burst, dem, gamma0, phase_noise,
-                               amb_height, offpoly, margin)
+                               amb_height, offpoly, margin, sec_seed)
+        if aps_sec is not None:
+            sec = sec * np.exp(1j * np.asarray(aps_sec, np.float64))
         out["ref_slc"] = ref.astype(np.complex64)
         out["sec_slc"] = sec.astype(np.complex64)
     else:
-        out.update(_slcs_fast(rng, orb_ref, orb_sec, burst, dem, gamma0, offpoly, device))
+        out.update(_slcs_fast(rng, orb_ref, orb_sec, burst, dem, gamma0, offpoly, device,
+                              sec_seed, aps_sec))
     return out
 
 
@@ -402,14 +480,15 @@ def _sec_positions(burst, offpoly, ii, jj):
 
 
 def _slcs_exact(rng, orb_ref, orb_sec, burst, dem, gamma0, phase_noise, amb_height, offpoly,
-                margin):
+                margin, sec_seed=None):
     L, N = burst["lines"], burst["samples"]
     R0 = burst["slant_range_time"] * SOL / 2
     m = margin
     shape = (L + 2 * m, N + 2 * m)
     z = bandlimited_noise(rng, shape)
-    n = bandlimited_noise(rng, shape)
-    eps = np.real(bandlimited_noise(rng, shape, 0.3, 0.3)) * phase_noise
+    rs = rng if sec_seed is None else np.random.default_rng(sec_seed)
+    n = bandlimited_noise(rs, shape)
+    eps = np.real(bandlimited_noise(rs, shape, 0.3, 0.3)) * phase_noise
     ref = z[m:m + L, m:m + N] * np.exp(1j * ramp(orb_ref, burst))
     ii, jj = np.meshgrid(np.arange(L), np.arange(N), indexing="ij")
     ia, ja = _sec_positions(burst, offpoly, ii, jj)
@@ -434,7 +513,8 @@ def _slcs_exact(rng, orb_ref, orb_sec, burst, dem, gamma0, phase_noise, amb_heig
     return ref, sec
 
 
-def _slcs_fast(rng, orb_ref, orb_sec, burst, dem, gamma0, offpoly, device):
+def _slcs_fast(rng, orb_ref, orb_sec, burst, dem, gamma0, offpoly, device, sec_seed=None,
+               aps_sec=None):
     import torch
     import torch.nn.functional as F
     L, N = burst["lines"], burst["samples"]
@@ -443,9 +523,14 @@ def _slcs_fast(rng, orb_ref, orb_sec, burst, dem, gamma0, offpoly, device):
     gen = torch.Generator(device=dev)
     gen.manual_seed(int(rng.integers(1 << 31)))
 
-    def noise():
-        w = torch.complex(torch.randn(L, N, device=dev, generator=gen),
-                          torch.randn(L, N, device=dev, generator=gen)) / math.sqrt(2)
+    gen2 = gen
+    if sec_seed is not None:
+        gen2 = torch.Generator(device=dev)
+        gen2.manual_seed(int(sec_seed))
+
+    def noise(g=gen):
+        w = torch.complex(torch.randn(L, N, device=dev, generator=g),
+                          torch.randn(L, N, device=dev, generator=g)) / math.sqrt(2)
         fa = torch.fft.fftfreq(L, device=dev).abs()[:, None] * 2
         fr = torch.fft.fftfreq(N, device=dev).abs()[None, :] * 2
         win = ((fa < 0.7) * 0.5 * (1 + torch.cos(math.pi * torch.clamp(fa / 0.7, max=1)))
@@ -454,7 +539,7 @@ def _slcs_fast(rng, orb_ref, orb_sec, burst, dem, gamma0, offpoly, device):
         return zz / torch.sqrt(torch.mean(zz.abs() ** 2))
 
     z = noise()
-    n2 = noise()
+    n2 = noise(gen2)
     # coarse secondary mapping (exact geometry), upsampled on the GPU
     cs, cr = max(L // 24, 2), max(N // 96, 2)
     ic = np.linspace(0, L - 1, cs)
@@ -488,6 +573,9 @@ def _slcs_fast(rng, orb_ref, orb_sec, burst, dem, gamma0, offpoly, device):
     rmp = torch.polar(torch.ones_like(ph), ph)
     ref = (z * rmp).to(torch.complex64)
     phs = torch.remainder(-4 * math.pi * fine[2] / WAVELENGTH, 2 * math.pi).float()
+    if aps_sec is not None:
+        aps = torch.as_tensor(aps_sec, device=dev).float()
+        phs = torch.remainder(phs + aps, 2 * math.pi)
     sec = (gamma0 * zs + math.sqrt(1 - gamma0 ** 2) * ns) * torch.polar(torch.ones_like(phs), phs)
     sec = (sec * rmp).to(torch.complex64)
     return {"ref_slc": ref.contiguous(), "sec_slc": sec.contiguous()}
