@@ -1,0 +1,148 @@
+// Goldstein overlap-add without a patch buffer (shared by the per-image and
+// batched kernels).
+//
+// Work item = one patch row pi (a band of 32 image rows) x one chunk of
+// kBandCW consecutive patch columns, processed by one wavefront: it filters
+// the chunk's patches in column order on MFMA (gold_mfma_dev.h) and adds
+// each weighted patch straight from registers into its own 32 x 64 circular
+// LDS buffer holding the band's running overlap-add.  After patch j, the
+// columns left of patch j+1's origin are final for the (band, chunk) and are
+// written to a partial buffer of 32 rows x kBandCols per item.  The output
+// pixel (y, x) is the sum of the partials of the (at most 4) bands and (at
+// most 2) chunks that cover it, in increasing band then chunk order, over
+// the triangular weight sum (goldstein.cu:257-292).  Every sum has a fixed
+// order, so results are bit-reproducible; the partials cost 4 x (1 + 24 /
+// (8 kBandCW)) times the output in traffic instead of the 16 x of a patch
+// buffer.
+#pragma once
+
+#include "gold_dev.h"
+#include "gold_mfma_dev.h"
+
+namespace s1ifg {
+
+constexpr int kBandCW = 16;                    // patches per chunk
+constexpr int kBandCols = 8 * kBandCW + 24;    // partial width (columns)
+constexpr int kBandBuf = 64;                   // circular buffer width
+constexpr int kBandLd = kBandBuf + 1;          // padded row stride
+constexpr int kBandWaves = 4;                  // wavefronts per block
+
+struct BandLds {
+  float2 buf[kBandWaves][kGoldN * kBandLd];
+};
+
+__device__ __forceinline__ int gold_origin(int j, int n) { return min(j * (kGoldN / 4), n - kGoldN); }
+
+__host__ __device__ __forceinline__ int band_nchunk(int pj_n) { return (pj_n + kBandCW - 1) / kBandCW; }
+
+// One (band, chunk) item on one wavefront (no block barriers).  buf: the
+// wave's LDS buffer (kGoldN x kBandLd); partial: this image's partial buffer
+// [pi_n][nchunk][32][kBandCols].
+__device__ __forceinline__ void band_item(const mf::Consts &K, const float2 *__restrict__ img,
+                                          int nrow, int ncol, int pi_n, int pj_n, int pi, int c,
+                                          float alpha, float2 *__restrict__ partial,
+                                          float2 *buf) {
+  const int lane = threadIdx.x & 63;
+  const int h = lane >> 5, n = lane & 31;
+  const int nchunk = band_nchunk(pj_n);
+  const int j0 = c * kBandCW;
+  const int j1 = min(j0 + kBandCW, pj_n);
+  const int ox0 = gold_origin(j0, ncol);
+  float2 *dst = partial + (size_t)(pi * nchunk + c) * kGoldN * kBandCols;
+  for (int i = lane; i < kGoldN * kBandLd; i += 64) buf[i] = make_float2(0.f, 0.f);
+  __builtin_amdgcn_wave_barrier();
+  int lo = ox0;
+  const long rowbase = (long)pi * pj_n;
+  float2 nxt[16];
+  mf::load_patch(img, nrow, ncol, pj_n, rowbase + j0, h, n, nxt);
+  for (int j = j0; j < j1; ++j) {
+    float2 x[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x[r] = nxt[r];
+    if (j + 1 < j1) mf::load_patch(img, nrow, ncol, pj_n, rowbase + j + 1, h, n, nxt);
+    float2 out[16];
+    mf::filter_patch_regs(K, x, h, n, alpha, out);
+    // add the patch (each lane owns distinct (row, column) entries)
+    const int ox = gold_origin(j, ncol);
+    float2 *row = buf + n * kBandLd;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        float2 *e = row + ((ox + 4 * h + 8 * q + t) & (kBandBuf - 1));
+        float2 v = *e;
+        v.x += out[4 * q + t].x;
+        v.y += out[4 * q + t].y;
+        *e = v;
+      }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // columns no later patch of the chunk reaches are final
+    const int done = (j + 1 < j1) ? gold_origin(j + 1, ncol) : ox + kGoldN;
+    const int w = done - lo;
+    for (int i = lane; i < w * kGoldN; i += 64) {
+      const int r = i / w, col = lo + (i - r * w);
+      float2 *e = buf + r * kBandLd + (col & (kBandBuf - 1));
+      dst[(size_t)r * kBandCols + (col - ox0)] = *e;
+      *e = make_float2(0.f, 0.f);
+    }
+    lo = done;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// Output pixel (y, x): partial sums of the bands and chunks covering it
+// over the triangular weight sum; 0 where the weight sum is 0.
+__device__ __forceinline__ float2 band_sum_pixel(const float2 *__restrict__ partial, int nrow,
+                                                 int ncol, int pi_n, int pj_n, int y, int x) {
+  const int step = kGoldN / 4;
+  const int nchunk = band_nchunk(pj_n);
+  // weight sum: sum over covering patches (pi, pj) of wx + wy
+  float cy = 0.f, sy = 0.f, cx = 0.f, sx = 0.f;
+  const int pi_lo = y - (kGoldN - 1) > 0 ? (y - (kGoldN - 1) + step - 1) / step : 0;
+  const int pj_lo = x - (kGoldN - 1) > 0 ? (x - (kGoldN - 1) + step - 1) / step : 0;
+  for (int pi = pi_lo; pi < pi_n; ++pi) {
+    const int o = gold_origin(pi, nrow);
+    if (o > y) break;
+    if (y >= o + kGoldN) continue;
+    const int ty = y - o;
+    cy += 1.f;
+    sy += (float)(ty < kGoldN / 2 ? ty : kGoldN - 1 - ty);
+  }
+  for (int pj = pj_lo; pj < pj_n; ++pj) {
+    const int o = gold_origin(pj, ncol);
+    if (o > x) break;
+    if (x >= o + kGoldN) continue;
+    const int tx = x - o;
+    cx += 1.f;
+    sx += (float)(tx < kGoldN / 2 ? tx : kGoldN - 1 - tx);
+  }
+  const float ws = cx * sy + cy * sx;
+  if (!(ws > 0.f)) return make_float2(0.f, 0.f);
+  // last patch column with origin <= x, and the chunks that can cover x
+  const int jx = x >= gold_origin(pj_n - 1, ncol) ? pj_n - 1 : min(x / step, pj_n - 1);
+  const int c_hi = jx / kBandCW;
+  const int c_lo = c_hi > 0 ? c_hi - 1 : 0;
+  float ax = 0.f, ay = 0.f;
+  for (int pi = pi_lo; pi < pi_n; ++pi) {
+    const int o = gold_origin(pi, nrow);
+    if (o > y) break;
+    if (y >= o + kGoldN) continue;
+    const int ty = y - o;
+    for (int c = c_lo; c <= c_hi && c < nchunk; ++c) {
+      const int a = gold_origin(c * kBandCW, ncol);
+      const int b = gold_origin(min(c * kBandCW + kBandCW, pj_n) - 1, ncol) + kGoldN;
+      if (x < a || x >= b) continue;
+      const float2 v =
+          partial[((size_t)(pi * nchunk + c) * kGoldN + ty) * kBandCols + (x - a)];
+      ax += v.x;
+      ay += v.y;
+    }
+  }
+  return make_float2(ax / ws, ay / ws);
+}
+
+}  // namespace s1ifg

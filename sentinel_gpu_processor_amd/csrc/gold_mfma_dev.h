@@ -188,10 +188,11 @@ __device__ __forceinline__ void load_patch(const float2 *__restrict__ img, int n
     for (int t = 0; t < 4; ++t) v[4 * q + t] = src[8 * q + t];
 }
 
-// Filter one patch held in x (lane layout) and store the weighted result
-// w(row, col) z'/n^2 into dst (32 x 32 row-major).
-__device__ __forceinline__ void filter_patch(const Consts &K, const float2 x[16], int h, int n,
-                                             float alpha, float2 *__restrict__ dst) {
+// Filter one patch held in x (lane layout) into its weighted result
+// w(row, col) z'/n^2 in the same layout: lane (h, n) holds row n, columns
+// 4h + 8q + t (q, t = 0..3) in out[4q + t].
+__device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 x[16], int h,
+                                                  int n, float alpha, float2 out[16]) {
   float xr[16], xi[16];
   float mx = 0.f;
 #pragma unroll
@@ -270,7 +271,6 @@ __device__ __forceinline__ void filter_patch(const Consts &K, const float2 x[16]
   // weighted output (goldstein.cu:257-267), undo the patch scale
   const float wy = (float)(n < kN / 2 ? n : kN - 1 - n);
   const float norm = 1.0f / (kN * kN * scale);
-  float2 *d = dst + n * kN + 4 * h;
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -278,7 +278,7 @@ __device__ __forceinline__ void filter_patch(const Consts &K, const float2 x[16]
       const int col = 4 * h + 8 * q + t;
       const float wx = (float)(col < kN / 2 ? col : kN - 1 - col);
       const float w = (wx + wy) * norm;
-      d[8 * q + t] = make_float2(zr[4 * q + t] * w, zi[4 * q + t] * w);
+      out[4 * q + t] = make_float2(zr[4 * q + t] * w, zi[4 * q + t] * w);
     }
 }
 

@@ -3,7 +3,7 @@
 // (rows, tiles or patches of every pair, pairs of uneven size back to back).
 // A work item is mapped to its pair by a binary search over the stage's
 // prefix array.  The per-item device code is the one the single-pair entry
-// points run (geom_dev.h, gold_dev.h, gold_mfma_dev.h, ifg_dev.h), so each
+// points run (geom_dev.h, gold_dev.h, gold_band_dev.h, ifg_dev.h), so each
 // pair's products are bit-identical to those entry points.
 //
 // The launch sequence can be captured once into a HIP graph (on the batch's
@@ -14,7 +14,7 @@
 
 #include "geom_dev.h"
 #include "gold_dev.h"
-#include "gold_mfma_dev.h"
+#include "gold_band_dev.h"
 #include "ifg_dev.h"
 
 namespace s1ifg {
@@ -130,40 +130,24 @@ __global__ __launch_bounds__(256) void kb_pair(const S1C PairDev *__restrict__ P
   }
 }
 
-constexpr int kGoldWaves = 4;
-
-// Goldstein patch pass on MFMA: persistent wavefronts over the patches of
-// every pair, the next patch prefetched (no block barriers).
-__global__ __launch_bounds__(256) void kb_gold_mfma(const S1C PairDev *__restrict__ P,
-                                                    const S1C long *__restrict__ off, int n,
-                                                    float alpha) {
-  const int lane = threadIdx.x & 63;
-  const int h = lane >> 5, nn = lane & 31;
+// Goldstein FP16 split: (band, chunk) items of every pair on persistent
+// wavefronts, overlap-add fused (gold_band_dev.h).
+__global__ __launch_bounds__(64 * kBandWaves) void kb_gold_band(
+    const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, float alpha) {
+  __shared__ BandLds L;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long total = off[n];
-  const long nwaves = (long)gridDim.x * kGoldWaves;
-  long patch = (long)blockIdx.x * kGoldWaves + (threadIdx.x >> 6);
-  if (patch >= total) return;
+  long t = (long)blockIdx.x * kBandWaves + wave;
+  if (t >= total) return;
   mf::Consts K;
-  mf::build_consts(h, nn, K);
-  int p = find_item(off, n, patch);
-  float2 nxt[16];
-  {
+  mf::build_consts(lane >> 5, lane & 31, K);
+  for (; t < total; t += (long)gridDim.x * kBandWaves) {
+    const int p = find_item(off, n, t);
     const PairDev &d = *(const PairDev *)&P[p];
-    mf::load_patch(d.ifg, d.nr_ml, d.nc_ml, d.pj_n, patch - off[p], h, nn, nxt);
-  }
-  for (; patch < total; patch += nwaves) {
-    float2 x[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) x[r] = nxt[r];
-    const int pc = p;
-    const long next = patch + nwaves;
-    if (next < total) {
-      p = find_item(off, n, next);
-      const PairDev &d = *(const PairDev *)&P[p];
-      mf::load_patch(d.ifg, d.nr_ml, d.nc_ml, d.pj_n, next - off[p], h, nn, nxt);
-    }
-    const PairDev &dc = *(const PairDev *)&P[pc];
-    mf::filter_patch(K, x, h, nn, alpha, dc.patches + (size_t)(patch - off[pc]) * (mf::kN * mf::kN));
+    const long lt = t - off[p];
+    const int nchunk = band_nchunk(d.pj_n);
+    band_item(K, d.ifg, d.nr_ml, d.nc_ml, d.pi_n, d.pj_n, (int)(lt / nchunk), (int)(lt % nchunk),
+              alpha, d.patches, L.buf[wave]);
   }
 }
 
@@ -193,7 +177,8 @@ __global__ __launch_bounds__(128) void kb_gold_fp32(const S1C PairDev *__restric
 
 // Overlap-add gather over 64 x 4 tiles of every filtered image.
 __global__ __launch_bounds__(256) void kb_gather(const S1C PairDev *__restrict__ P,
-                                                 const S1C long *__restrict__ off, int n) {
+                                                 const S1C long *__restrict__ off, int n,
+                                                 int band) {
   const long total = off[n];
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
     const int p = find_item(off, n, t);
@@ -204,7 +189,8 @@ __global__ __launch_bounds__(256) void kb_gather(const S1C PairDev *__restrict__
     const int x = tx * 64 + threadIdx.x, y = ty * 4 + threadIdx.y;
     if (x < d.nc_ml && y < d.nr_ml)
       d.filt[(size_t)y * d.nc_ml + x] =
-          gold_gather_pixel(d.patches, d.nr_ml, d.nc_ml, d.pi_n, d.pj_n, y, x);
+          band ? band_sum_pixel(d.patches, d.nr_ml, d.nc_ml, d.pi_n, d.pj_n, y, x)
+               : gold_gather_pixel(d.patches, d.nr_ml, d.nc_ml, d.pi_n, d.pj_n, y, x);
   }
 }
 
@@ -330,8 +316,8 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
   }
   if (long t = total(kStGold)) {
     if (o.gold_mode == S1IFG_GOLD_FP16_SPLIT)
-      hipLaunchKernelGGL(kb_gold_mfma, dim3(grid_of(t, kGoldWaves, ncu * o.gold_blocks_per_cu)),
-                         dim3(64 * kGoldWaves), 0, s, pairs, off[kStGold], n, o.alpha);
+      hipLaunchKernelGGL(kb_gold_band, dim3(grid_of(t, kBandWaves, ncu * o.gold_blocks_per_cu)),
+                         dim3(64 * kBandWaves), 0, s, pairs, off[kStGold], n, o.alpha);
     else
       hipLaunchKernelGGL(kb_gold_fp32, dim3(grid_of(t, kGoldGroups, ncu * 8 * o.gold_blocks_per_cu)),
                          dim3(32 * kGoldGroups), 0, s, pairs, off[kStGold], n, o.alpha);
@@ -339,7 +325,7 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
   }
   if (long t = total(kStGather)) {
     hipLaunchKernelGGL(kb_gather, dim3(grid_of(t, 1, ncu * o.stream_blocks_per_cu)), dim3(64, 4),
-                       0, s, pairs, off[kStGather], n);
+                       0, s, pairs, off[kStGather], n, o.gold_mode == S1IFG_GOLD_FP16_SPLIT);
     S1_LAUNCH_CHECK("kb_gather");
   }
   if (long t = total(kStCoh)) {
@@ -447,7 +433,9 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
       if (d.filt == d.ifg) return set_error(S1IFG_EINVAL, "pair %d: in-place filtering is not supported", p);
       goldstein_grid(d.nr_ml, d.nc_ml, &d.pi_n, &d.pj_n);
       patch_off[p] = patch_bytes;
-      patch_bytes += align256((size_t)d.pi_n * d.pj_n * kGoldN * kGoldN * sizeof(float2));
+      patch_bytes += align256(o.gold_mode == S1IFG_GOLD_FP16_SPLIT
+                                  ? (size_t)d.pi_n * band_nchunk(d.pj_n) * kGoldN * kBandCols * sizeof(float2)
+                                  : (size_t)d.pi_n * d.pj_n * kGoldN * kGoldN * sizeof(float2));
     }
     tab_off[p] = tab_bytes;
     tab_bytes += align256(sizeof(RowTab) * (size_t)d.grid.nrow) +
@@ -469,7 +457,10 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
     const PairDev &d = b->pairs[p];
     long tab = d.grid.nrow > d.grid.ncol ? d.grid.nrow : d.grid.ncol;
     b->off[kStTables][p + 1] = b->off[kStTables][p] + tab;
-    b->off[kStGold][p + 1] = b->off[kStGold][p] + (d.filt ? (long)d.pi_n * d.pj_n : 0);
+    const long gitems = o.gold_mode == S1IFG_GOLD_FP16_SPLIT
+                            ? (long)d.pi_n * band_nchunk(d.pj_n)
+                            : (long)d.pi_n * d.pj_n;
+    b->off[kStGold][p + 1] = b->off[kStGold][p] + (d.filt ? gitems : 0);
     long gtiles = (long)((d.nc_ml + 63) / 64) * ((d.nr_ml + 3) / 4);
     b->off[kStGather][p + 1] = b->off[kStGather][p] + (d.filt ? gtiles : 0);
     long ctiles = (long)((d.nc_ml + kCohTX - 1) / kCohTX) * ((d.nr_ml + kCohTY - 1) / kCohTY);

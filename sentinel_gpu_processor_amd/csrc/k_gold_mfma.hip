@@ -1,36 +1,34 @@
-// Goldstein patch pass on MFMA for one image (the filter itself is in
-// gold_mfma_dev.h).
-#include "gold_mfma_dev.h"
+// Goldstein filter in FP16 hi/lo split on MFMA with the overlap-add fused
+// into the patch pass (gold_band_dev.h): the filtered patches never leave
+// the chip; per-band partial sums do.
+#include "gold_band_dev.h"
 
 namespace s1ifg {
 namespace {
 
-constexpr int kWaves = 4;
-
-// Persistent wavefronts: each wave builds the constant operands once and
-// filters patches wave_id, wave_id + nwaves, ... with the next patch
-// prefetched (the loop bound is uniform per wavefront; no block barriers).
-__global__ __launch_bounds__(256) void k_gold_mfma_patches(const float2 *__restrict__ img,
-                                                           int nrow, int ncol, int pi_n,
-                                                           int pj_n, float alpha,
-                                                           float2 *__restrict__ patches) {
-  const int lane = threadIdx.x & 63;
-  const int h = lane >> 5, n = lane & 31;
-  const long total = (long)pi_n * pj_n;
-  const long nwaves = (long)gridDim.x * kWaves;
-  long patch = (long)blockIdx.x * kWaves + (threadIdx.x >> 6);
-  if (patch >= total) return;
+// Persistent wavefronts over the (band, chunk) items (no block barriers).
+__global__ __launch_bounds__(64 * kBandWaves) void k_gold_band(
+    const float2 *__restrict__ img, int nrow, int ncol, int pi_n, int pj_n, float alpha,
+    float2 *__restrict__ partial) {
+  __shared__ BandLds L;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nchunk = band_nchunk(pj_n);
+  const long total = (long)pi_n * nchunk;
+  long t = (long)blockIdx.x * kBandWaves + wave;
+  if (t >= total) return;
   mf::Consts K;
-  mf::build_consts(h, n, K);
-  float2 nxt[16];
-  mf::load_patch(img, nrow, ncol, pj_n, patch, h, n, nxt);
-  for (; patch < total; patch += nwaves) {
-    float2 x[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) x[r] = nxt[r];
-    if (patch + nwaves < total) mf::load_patch(img, nrow, ncol, pj_n, patch + nwaves, h, n, nxt);
-    mf::filter_patch(K, x, h, n, alpha, patches + (size_t)patch * (mf::kN * mf::kN));
-  }
+  mf::build_consts(lane >> 5, lane & 31, K);
+  for (; t < total; t += (long)gridDim.x * kBandWaves)
+    band_item(K, img, nrow, ncol, pi_n, pj_n, (int)(t / nchunk), (int)(t % nchunk), alpha,
+              partial, L.buf[wave]);
+}
+
+__global__ void k_gold_band_sum(const float2 *__restrict__ partial, int nrow, int ncol, int pi_n,
+                                int pj_n, float2 *__restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= ncol || y >= nrow) return;
+  out[(size_t)y * ncol + x] = band_sum_pixel(partial, nrow, ncol, pi_n, pj_n, y, x);
 }
 
 }  // namespace
@@ -48,15 +46,23 @@ int device_cu_count(int device) {
   return ncu[dev];
 }
 
-int goldstein_mfma(s1ifg_ctx *ctx, const float2 *in, int nrow, int ncol, float alpha, int pi_n,
-                   int pj_n, float2 *patches) {
-  long total = (long)pi_n * pj_n;
-  long want = (total + kWaves - 1) / kWaves;
-  long cap = 2L * device_cu_count(ctx->device);
-  unsigned blocks = (unsigned)(want < cap ? want : cap);
-  hipLaunchKernelGGL(k_gold_mfma_patches, dim3(blocks), dim3(64 * kWaves), 0, ctx->stream, in,
-                     nrow, ncol, pi_n, pj_n, alpha, patches);
-  S1_LAUNCH_CHECK("k_gold_mfma_patches");
+size_t band_partial_bytes(int pi_n, int pj_n) {
+  return (size_t)pi_n * band_nchunk(pj_n) * kGoldN * kBandCols * sizeof(float2);
+}
+
+int goldstein_band(s1ifg_ctx *ctx, const float2 *in, int nrow, int ncol, float alpha, int pi_n,
+                   int pj_n, float2 *partial, float2 *out) {
+  const long items = (long)pi_n * band_nchunk(pj_n);
+  const long want = (items + kBandWaves - 1) / kBandWaves;
+  const long cap = 2L * device_cu_count(ctx->device);
+  const unsigned blocks = (unsigned)(want < cap ? want : cap);
+  hipLaunchKernelGGL(k_gold_band, dim3(blocks), dim3(64 * kBandWaves), 0, ctx->stream, in, nrow, ncol, pi_n,
+                     pj_n, alpha, partial);
+  S1_LAUNCH_CHECK("k_gold_band");
+  dim3 blk(64, 4), grd((ncol + 63) / 64, (nrow + 3) / 4);
+  hipLaunchKernelGGL(k_gold_band_sum, grd, blk, 0, ctx->stream, partial, nrow, ncol, pi_n, pj_n,
+                     out);
+  S1_LAUNCH_CHECK("k_gold_band_sum");
   return S1IFG_OK;
 }
 

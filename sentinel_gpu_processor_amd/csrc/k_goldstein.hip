@@ -1,12 +1,12 @@
 // Goldstein adaptive filter (goldstein.cu:87-408), deterministic form.
+// FP16 split mode: the fused band kernels of k_gold_mfma.hip.  FP32 mode:
 //
 // Pass 1: every 32x32 patch (origin min(8 p, n - 32), goldstein.cu:334-342)
 // is filtered -- forward 2-D transform, 7x7 circular smoothing of |Z|
 // (goldstein.cu:87-123), patch-mean normalisation (142-225), Z *= S^alpha
 // where positive (125-140), inverse transform -- and stored multiplied by
 // the triangular weight w = wx + wy over n^2 (227-274) in a workspace
-// buffer.  Two implementations: FP32 radix-2 FFTs (gold_dev.h) and the FP16
-// hi/lo split DFT-by-GEMM on MFMA (gold_mfma_dev.h).
+// buffer, with FP32 radix-2 FFTs (gold_dev.h).
 // Pass 2 (one thread per output pixel): sums the contributions of every
 // patch covering the pixel in a fixed order and divides by the weight sum
 // (280-292); pixels with weight sum 0 are 0.
@@ -59,8 +59,9 @@ int goldstein_grid(int nrow, int ncol, int *pi_n, int *pj_n) {
   return 0;
 }
 
-int goldstein_mfma(s1ifg_ctx *ctx, const float2 *in, int nrow, int ncol, float alpha, int pi_n,
-                   int pj_n, float2 *patches);
+size_t band_partial_bytes(int pi_n, int pj_n);
+int goldstein_band(s1ifg_ctx *ctx, const float2 *in, int nrow, int ncol, float alpha, int pi_n,
+                   int pj_n, float2 *partial, float2 *out);
 
 }  // namespace s1ifg
 
@@ -81,17 +82,17 @@ extern "C" int s1ifg_goldstein(s1ifg_ctx *ctx, const s1ifg_cf32 *in, int nrow, i
   goldstein_grid(nrow, ncol, &pi_n, &pj_n);
   size_t npatch = (size_t)pi_n * pj_n;
   void *ws = nullptr;
+  if (mode == S1IFG_GOLD_FP16_SPLIT) {
+    if ((rc = ctx_workspace(ctx, band_partial_bytes(pi_n, pj_n), &ws))) return rc;
+    return goldstein_band(ctx, (const float2 *)in, nrow, ncol, alpha, pi_n, pj_n, (float2 *)ws,
+                          (float2 *)out);
+  }
   if ((rc = ctx_workspace(ctx, npatch * kGoldN * kGoldN * sizeof(float2), &ws))) return rc;
   float2 *patches = (float2 *)ws;
-  if (mode == S1IFG_GOLD_FP32) {
-    unsigned blocks = (unsigned)((npatch + kPatchesPerBlock - 1) / kPatchesPerBlock);
-    hipLaunchKernelGGL(k_gold_fp32_patches, dim3(blocks), dim3(128), 0, ctx->stream,
-                       (const float2 *)in, nrow, ncol, pi_n, pj_n, alpha, patches);
-    S1_LAUNCH_CHECK("k_gold_fp32_patches");
-  } else {
-    if ((rc = goldstein_mfma(ctx, (const float2 *)in, nrow, ncol, alpha, pi_n, pj_n, patches)))
-      return rc;
-  }
+  unsigned blocks = (unsigned)((npatch + kPatchesPerBlock - 1) / kPatchesPerBlock);
+  hipLaunchKernelGGL(k_gold_fp32_patches, dim3(blocks), dim3(128), 0, ctx->stream,
+                     (const float2 *)in, nrow, ncol, pi_n, pj_n, alpha, patches);
+  S1_LAUNCH_CHECK("k_gold_fp32_patches");
   dim3 blk(64, 4), grd((ncol + 63) / 64, (nrow + 3) / 4);
   hipLaunchKernelGGL(k_gold_gather, grd, blk, 0, ctx->stream, patches, nrow, ncol, pi_n, pj_n,
                      (float2 *)out);

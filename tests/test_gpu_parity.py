@@ -184,7 +184,7 @@ GOLD_TOL = {"fp32": (5e-5, 1e-4), "fp16_split": (2e-4, 3e-4)}
 
 
 @pytest.mark.parametrize("mode", ["fp32", "fp16_split"])
-@pytest.mark.parametrize("shape", [(64, 96), (77, 130), (32, 32)])
+@pytest.mark.parametrize("shape", [(64, 96), (77, 130), (32, 32), (70, 605)])
 def test_goldstein(ctx, shape, mode):
     from sentinel_gpu_processor_amd import kernels as K
     rng = np.random.default_rng(5)
