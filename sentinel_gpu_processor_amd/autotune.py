@@ -18,7 +18,7 @@ CANDIDATES = {
     "pair_block_x": (32, 64, 128),
     "pair_blocks_per_cu": (3, 4, 6, 8, 16),
     "gold_blocks_per_cu": (1, 2, 3, 4),
-    "stream_blocks_per_cu": (4, 8, 16, 32),
+    "stream_blocks_per_cu": (4, 8, 16, 32, 64),
 }
 
 

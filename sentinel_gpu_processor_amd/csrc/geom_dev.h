@@ -160,19 +160,18 @@ __device__ __forceinline__ void sinc_weights(float f, float w[8]) {
   for (int i = 0; i < 8; ++i) w[i] *= inv;
 }
 
-// Geocode one target pixel of one burst (reproject, geo2rdr.cu:194-247).
-// Returns the complex value (0 outside the validity window).  rgoff/azoff
-// are the sampling positions (offset polynomial applied), rng the geometric
-// range.
+// Geometry and phase of one target pixel of one burst (reproject,
+// geo2rdr.cu:194-219): the sampling position rg/az (offset polynomial
+// applied), the geometric range, and sin/cos of the flattening + reramp
+// phase.  Returns false outside the validity window (the phase is then not
+// computed).
 template <int INTERP>
-__device__ __forceinline__ float2 geocode_pixel(const GeomDev &g, const OffPoly &op,
-                                                const double X[3], const float2 *__restrict__ z,
-                                                double *rg_out = nullptr,
-                                                double *az_out = nullptr,
-                                                double *rng_out = nullptr) {
+__device__ __forceinline__ bool geocode_geom(const GeomDev &g, const OffPoly &op,
+                                             const double X[3], double &rg, double &az,
+                                             double &rng, float &sp, float &cp) {
   double t, dr[3];
   zero_doppler(g, X, t, dr);
-  double rng = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+  rng = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
   double rgoff = (rng - g.rngstart) * g.inv_dmrg;
   double azoff = (t - g.tstart) * g.inv_dtaz;
   double drg = 0.0, daz = 0.0;
@@ -181,16 +180,14 @@ __device__ __forceinline__ float2 geocode_pixel(const GeomDev &g, const OffPoly 
     drg = op.p[0] + op.p[1] * xs + op.p[2] * ys;
     daz = op.p[3] + op.p[4] * xs + op.p[5] * ys;
   }
-  double rg = rgoff + drg, az = azoff + daz;
-  if (rg_out) *rg_out = rg;
-  if (az_out) *az_out = az;
-  if (rng_out) *rng_out = rng;
+  rg = rgoff + drg;
+  az = azoff + daz;
   bool valid;
   if (INTERP == S1IFG_INTERP_BILINEAR)
     valid = rg > g.fvs && rg < g.lvs - 1 && az >= g.flv + 5 && az < g.llv - 5;
   else
     valid = rg >= g.fvs + 3 && rg < g.lvs - 4 && az >= g.flv + 5 && az < g.llv - 5;
-  if (!valid) return make_float2(0.f, 0.f);
+  if (!valid) return false;
   // reramp (geo2rdr.cu:210-219) at the sampling position
   double rp = rng + drg * g.dmrg;
   double dt = rp * 2 / kSol - g.fm[0];
@@ -204,9 +201,52 @@ __device__ __forceinline__ float2 geocode_pixel(const GeomDev &g, const OffPoly 
   double etadiff = eta - etaref;
   double reramp = -kPi * kt * etadiff * etadiff;
   float ph = reduce_phase(4.0 * kPi / g.wvl * rng - reramp);
-  float sp, cp;
   sincosf(ph, &sp, &cp);
-  float rx, ry;
+  return true;
+}
+
+// 8 x 8 sinc interpolation of the deramped burst at fractional offsets
+// (fr, fa) from the window whose top-left tap is base (row stride ld),
+// rotated by (cp, sp).
+__device__ __forceinline__ float2 sinc_interp(const float2 *__restrict__ base, long ld, float fr,
+                                              float fa, float sp, float cp) {
+  float wr[8], wa[8];
+  sinc_weights(fr, wr);
+  sinc_weights(fa, wa);
+  float rx = 0.f, ry = 0.f;
+#pragma unroll
+  for (int ia = 0; ia < 8; ++ia) {
+    const float2 *row = base + ia * ld;
+    float ax = 0.f, ay = 0.f;
+#pragma unroll
+    for (int ir = 0; ir < 8; ++ir) {
+      float2 v = row[ir];
+      ax = fmaf(wr[ir], v.x, ax);
+      ay = fmaf(wr[ir], v.y, ay);
+    }
+    rx = fmaf(wa[ia], ax, rx);
+    ry = fmaf(wa[ia], ay, ry);
+  }
+  return make_float2(rx * cp - ry * sp, rx * sp + ry * cp);
+}
+
+// Geocode one target pixel of one burst (reproject, geo2rdr.cu:194-247).
+// Returns the complex value (0 outside the validity window).  rgoff/azoff
+// are the sampling positions (offset polynomial applied), rng the geometric
+// range.
+template <int INTERP>
+__device__ __forceinline__ float2 geocode_pixel(const GeomDev &g, const OffPoly &op,
+                                                const double X[3], const float2 *__restrict__ z,
+                                                double *rg_out = nullptr,
+                                                double *az_out = nullptr,
+                                                double *rng_out = nullptr) {
+  double rg, az, rng;
+  float sp, cp;
+  const bool valid = geocode_geom<INTERP>(g, op, X, rg, az, rng, sp, cp);
+  if (rg_out) *rg_out = rg;
+  if (az_out) *az_out = az;
+  if (rng_out) *rng_out = rng;
+  if (!valid) return make_float2(0.f, 0.f);
   const int N = g.samples;
   if (INTERP == S1IFG_INTERP_BILINEAR) {
     // geo2rdr.cu:220-237: range blends rounded to float, azimuth blend FP64
@@ -220,32 +260,14 @@ __device__ __forceinline__ float2 geocode_pixel(const GeomDev &g, const OffPoly 
     float c1y = (float)(b1.y * (1 - fr) + b2.y * fr);
     float c2x = (float)(b3.x * (1 - fr) + b4.x * fr);
     float c2y = (float)(b3.y * (1 - fr) + b4.y * fr);
-    rx = (float)(c1x * (1 - fa) + c2x * fa);
-    ry = (float)(c1y * (1 - fa) + c2y * fa);
-  } else {
-    int intr = (int)floor(rg);
-    int inta = (int)floor(az);
-    float wr[8], wa[8];
-    sinc_weights((float)(rg - intr), wr);
-    sinc_weights((float)(az - inta), wa);
-    rx = 0.f;
-    ry = 0.f;
-    const float2 *base = z + (size_t)(inta - 3) * N + (intr - 3);
-#pragma unroll
-    for (int ia = 0; ia < 8; ++ia) {
-      const float2 *row = base + (size_t)ia * N;
-      float ax = 0.f, ay = 0.f;
-#pragma unroll
-      for (int ir = 0; ir < 8; ++ir) {
-        float2 v = row[ir];
-        ax = fmaf(wr[ir], v.x, ax);
-        ay = fmaf(wr[ir], v.y, ay);
-      }
-      rx = fmaf(wa[ia], ax, rx);
-      ry = fmaf(wa[ia], ay, ry);
-    }
+    float rx = (float)(c1x * (1 - fa) + c2x * fa);
+    float ry = (float)(c1y * (1 - fa) + c2y * fa);
+    return make_float2(rx * cp - ry * sp, rx * sp + ry * cp);
   }
-  return make_float2(rx * cp - ry * sp, rx * sp + ry * cp);
+  int intr = (int)floor(rg);
+  int inta = (int)floor(az);
+  return sinc_interp(z + (size_t)(inta - 3) * N + (intr - 3), N, (float)(rg - intr),
+                     (float)(az - inta), sp, cp);
 }
 
 __device__ __forceinline__ void pixel_xyz(const GridDev &gr, const DemDev &dem,

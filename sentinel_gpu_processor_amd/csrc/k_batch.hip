@@ -7,8 +7,9 @@
 // pair's products are bit-identical to those entry points.
 //
 // The launch sequence can be captured once into a HIP graph (on the batch's
-// own stream) and replayed: every kernel writes all of its outputs, nothing
-// is cleared with memset, and the raw bursts are never written.
+// own stream) and replayed: every kernel writes all of its outputs, the
+// work-queue counters are zeroed by a kernel (no memset), and the raw bursts
+// are never written.
 #include <cstring>
 #include <vector>
 
@@ -16,6 +17,7 @@
 #include "gold_dev.h"
 #include "gold_band_dev.h"
 #include "ifg_dev.h"
+#include "pair_lds_dev.h"
 
 namespace s1ifg {
 
@@ -54,8 +56,12 @@ enum { kStTables, kStDeramp, kStPair, kStGold, kStGather, kStCoh, kNumStages };
 
 __device__ __forceinline__ const GeomDev &geo(const S1C GeomDev *g) { return *(const GeomDev *)g; }
 
-// Largest p in [0, n) with off[p] <= idx, for 0 <= idx < off[n].
-__device__ __forceinline__ int find_item(const S1C long *__restrict__ off, int n, long idx) {
+// Largest p in [0, n) with off[p] <= idx, for 0 <= idx < off[n]; hint is
+// checked first (the previous item's entry: consecutive items of a block
+// usually stay in one pair).
+__device__ __forceinline__ int find_item(const S1C long *__restrict__ off, int n, long idx,
+                                         int hint = -1) {
+  if (hint >= 0 && off[hint] <= idx && idx < off[hint + 1]) return hint;
   int lo = 0, hi = n;
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
@@ -78,25 +84,33 @@ __global__ void kb_tables(const S1C PairDev *__restrict__ P, const S1C long *__r
   }
 }
 
-// Deramp of every distinct burst of the batch; one block per burst line,
-// the element code of k_deramp.
+// Deramp of every distinct burst of the batch, the element code of
+// k_deramp.
+// Deramp work item: one burst line x kDerampChunk sample pairs (one pair
+// per thread).
+constexpr int kDerampChunk = 256;
+
 __global__ __launch_bounds__(256) void kb_deramp(const S1C ImgDev *__restrict__ I,
                                                  const S1C long *__restrict__ off, int nimg) {
   const long total = off[nimg];
+  int k = -1;
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
-    const int k = find_item(off, nimg, t);
+    k = find_item(off, nimg, t, k);
     const ImgDev &im = *(const ImgDev *)&I[k];
     const GeomDev &g = geo(im.g);
     const float2 *in = im.in;
     float2 *out = im.out;
-    const int i = (int)(t - off[k]);
     const int N = g.samples;
+    const int nchunk = (N + 2 * kDerampChunk - 1) / (2 * kDerampChunk);
+    const long lt = t - off[k];
+    const int i = (int)(lt / nchunk);
+    const int jj = (int)(lt - (long)i * nchunk) * kDerampChunk + threadIdx.x;
     const double eta = -g.lines * g.ati / 2. + i * g.ati;
     const size_t rowoff = (size_t)i * N;
     const bool vec = (N % 2 == 0) && (((uintptr_t)in | (uintptr_t)out) % 16 == 0);
     const float4 *in2 = (const float4 *)in;
     float4 *out2 = (float4 *)out;
-    for (int jj = threadIdx.x; jj < (N + 1) / 2; jj += blockDim.x) {
+    if (jj < (N + 1) / 2) {
       int j = 2 * jj;
       if (vec) {
         float4 a = in2[(rowoff >> 1) + jj];
@@ -117,8 +131,9 @@ __global__ __launch_bounds__(256) void kb_pair(const S1C PairDev *__restrict__ P
                                                const S1C long *__restrict__ off, int n, int rl,
                                                int cl) {
   const long total = off[n];
+  int ph = -1;
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
-    const int p = find_item(off, n, t);
+    const int p = ph = find_item(off, n, t, ph);
     const PairDev &d = *(const PairDev *)&P[p];
     const long lt = t - off[p];
     const int tiles_x = (d.nc_ml + blockDim.x - 1) / blockDim.x;
@@ -151,6 +166,44 @@ __global__ __launch_bounds__(64 * kBandWaves) void kb_gold_band(
   }
 }
 
+// Work-queue counters (one per stage), zeroed at the start of every run.
+__global__ void kb_reset(unsigned long long *ctr, int n) {
+  if ((int)threadIdx.x < n) ctr[threadIdx.x] = 0ull;
+}
+
+// Sinc pair kernel with LDS-staged windows over 16 x 16-cell tiles of every
+// pair (rl * cl <= kPairMaxPix).  Persistent blocks take tiles from a work
+// queue counter, so the tiles in flight on the chip stay together in the
+// bursts (cache reuse) whatever the grid size; the next tile is fetched
+// while the current one runs.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kb_pair_tiles(
+    const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, int rl, int cl,
+    unsigned long long *ctr) {
+  __shared__ PairLds L;
+  __shared__ long slot[2];
+  const long total = off[n];
+  if (threadIdx.x == 0) slot[0] = (long)atomicAdd(ctr, 1ull);
+  __syncthreads();
+  int cur = 0, p = -1;
+  for (;;) {
+    const long t = slot[cur];
+    if (t >= total) break;
+    // the next item is fetched while this one runs
+    long nxt = 0;
+    if (threadIdx.x == 0) nxt = (long)atomicAdd(ctr, 1ull);
+    p = find_item(off, n, t, p);
+    const PairDev &d = *(const PairDev *)&P[p];
+    const long lt = t - off[p];
+    const int tiles_x = (d.nc_ml + kPairTile - 1) / kPairTile;
+    const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
+    pair_tile_sinc(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
+                   d.sec_d, rl, cl, ty, tx, d.nr_ml, d.nc_ml, d.ifg, d.pr, d.ps, L);
+    if (threadIdx.x == 0) slot[cur ^ 1] = nxt;
+    cur ^= 1;
+    __syncthreads();
+  }
+}
+
 constexpr int kGoldGroups = 4;
 
 // FP32 Goldstein patch pass: 4 groups of 32 lanes per block, one patch each.
@@ -180,8 +233,9 @@ __global__ __launch_bounds__(256) void kb_gather(const S1C PairDev *__restrict__
                                                  const S1C long *__restrict__ off, int n,
                                                  int band) {
   const long total = off[n];
+  int ph = -1;
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
-    const int p = find_item(off, n, t);
+    const int p = ph = find_item(off, n, t, ph);
     const PairDev &d = *(const PairDev *)&P[p];
     const long lt = t - off[p];
     const int tiles_x = (d.nc_ml + 63) / 64;
@@ -201,8 +255,9 @@ __global__ __launch_bounds__(256) void kb_coherence(const S1C PairDev *__restric
                                                     int win, int fill, uint32_t seed) {
   __shared__ CohTile L;
   const long total = off[n];
+  int ph = -1;
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
-    const int p = find_item(off, n, t);
+    const int p = ph = find_item(off, n, t, ph);
     const PairDev &d = *(const PairDev *)&P[p];
     const long lt = t - off[p];
     const int tiles_x = (d.nc_ml + kCohTX - 1) / kCohTX;
@@ -238,6 +293,7 @@ struct s1ifg_batch {
   PairDev *d_pairs = nullptr;
   ImgDev *d_imgs = nullptr;
   long *d_off[kNumStages] = {};
+  unsigned long long *d_ctr = nullptr;  // work-queue counters, one per stage
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
@@ -260,9 +316,15 @@ int fill_defaults(s1ifg_batch_opts *o) {
   return S1IFG_OK;
 }
 
+bool use_pair_tiles(const s1ifg_batch_opts &o) {
+  return o.interp == S1IFG_INTERP_SINC && o.rl * o.cl <= kPairMaxPix;
+}
+
 // Pair-kernel tile prefix for the current tile width.
 void pair_prefix(s1ifg_batch *b) {
-  const int bx = b->opts.pair_block_x, by = 256 / bx;
+  const bool tiles = use_pair_tiles(b->opts);
+  const int bx = tiles ? kPairTile : b->opts.pair_block_x;
+  const int by = tiles ? kPairTile : 256 / bx;
   std::vector<long> &o = b->off[kStPair];
   o.assign(b->n + 1, 0);
   for (int p = 0; p < b->n; ++p) {
@@ -293,6 +355,8 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
   for (int st = 0; st < kNumStages; ++st) off[st] = (const S1C long *)b->d_off[st];
   const S1C PairDev *pairs = (const S1C PairDev *)b->d_pairs;
   auto total = [&](int st) { return b->off[st].back(); };
+  hipLaunchKernelGGL(kb_reset, dim3(1), dim3(64), 0, s, b->d_ctr, (int)kNumStages);
+  S1_LAUNCH_CHECK("kb_reset");
   if (long t = total(kStTables)) {
     hipLaunchKernelGGL(kb_tables, dim3(grid_of(t, 256, ncu * o.stream_blocks_per_cu)), dim3(256),
                        0, s, pairs, off[kStTables], n);
@@ -306,7 +370,10 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
   if (long t = total(kStPair)) {
     dim3 blk(o.pair_block_x, 256 / o.pair_block_x);
     dim3 grd(grid_of(t, 1, ncu * o.pair_blocks_per_cu));
-    if (o.interp == S1IFG_INTERP_BILINEAR)
+    if (use_pair_tiles(o))
+      hipLaunchKernelGGL(kb_pair_tiles, grd, dim3(256), 0, s, pairs, off[kStPair], n, o.rl,
+                         o.cl, b->d_ctr + kStPair);
+    else if (o.interp == S1IFG_INTERP_BILINEAR)
       hipLaunchKernelGGL(kb_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, s, pairs, off[kStPair],
                          n, o.rl, o.cl);
     else
@@ -452,7 +519,9 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
   // prefix arrays
   for (int st = 0; st < kNumStages; ++st) b->off[st].assign(st == kStDeramp ? nimg + 1 : n + 1, 0);
   for (int q = 0; q < nimg; ++q)
-    b->off[kStDeramp][q + 1] = b->off[kStDeramp][q] + img_geom[q]->dev.lines;
+    b->off[kStDeramp][q + 1] =
+        b->off[kStDeramp][q] + (long)img_geom[q]->dev.lines *
+                                   ((img_geom[q]->dev.samples + 2 * kDerampChunk - 1) / (2 * kDerampChunk));
   for (int p = 0; p < n; ++p) {
     const PairDev &d = b->pairs[p];
     long tab = d.grid.nrow > d.grid.ncol ? d.grid.nrow : d.grid.ncol;
@@ -478,6 +547,8 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
     off_pref[st] = sz;
     sz += align256(sizeof(long) * (2 * n + 1));
   }
+  size_t off_ctr = sz;
+  sz += align256(sizeof(unsigned long long) * kNumStages);
   size_t off_tab = sz;
   sz += tab_bytes;
   size_t off_patch = sz;
@@ -492,6 +563,7 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
   char *base = (char *)b->d_buf;
   b->d_pairs = (PairDev *)(base + off_pairs);
   b->d_imgs = (ImgDev *)(base + off_imgs);
+  b->d_ctr = (unsigned long long *)(base + off_ctr);
   for (int st = 0; st < kNumStages; ++st) b->d_off[st] = (long *)(base + off_pref[st]);
   for (int p = 0; p < n; ++p) {
     PairDev &d = b->pairs[p];

@@ -1,6 +1,7 @@
 // Geometry-path kernels: grid tables, orbit evaluation, deramp, geo2rdr,
 // geocode, valid-row scan, and the fused burst-pair kernel.
 #include "geom_dev.h"
+#include "pair_lds_dev.h"
 
 using namespace s1ifg;
 
@@ -105,6 +106,18 @@ __global__ void k_pair(const GeomDev *__restrict__ gpr, const GeomDev *__restric
   if (rm >= nr_ml || cm >= nc_ml) return;
   pair_ml_pixel<INTERP>(*gpr, *gps, ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, rm, cm, nc_ml,
                         ifg, pr, ps);
+}
+
+// Sinc pair kernel with LDS-staged burst windows: one 16 x 16 tile of look
+// cells per block (rl * cl <= kPairMaxPix).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pair_tiles(
+    const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr,
+    DemDev dem, const RowTab *rows, const ColTab *cols, const float *dh,
+    const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl, int cl, int nr_ml,
+    int nc_ml, float2 *__restrict__ ifg, float *__restrict__ pr, float *__restrict__ ps) {
+  __shared__ PairLds L;
+  pair_tile_sinc(*gpr, *gps, ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, blockIdx.y,
+                 blockIdx.x, nr_ml, nc_ml, ifg, pr, ps, L);
 }
 
 __global__ void k_rows_init(int *fl, int nrow) {
@@ -295,7 +308,13 @@ int s1ifg_pair(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
   ColTab *cols;
   if ((rc = grid_tables(ctx, gr, dd, &rows, &cols))) return rc;
   dim3 blk(64, 4), grd((nc + 63) / 64, (nr + 3) / 4);
-  if (interp == S1IFG_INTERP_BILINEAR)
+  if (interp == S1IFG_INTERP_SINC && rl * cl <= kPairMaxPix) {
+    dim3 tg((nc + kPairTile - 1) / kPairTile, (nr + kPairTile - 1) / kPairTile);
+    hipLaunchKernelGGL(k_pair_tiles, tg, dim3(256), 0, ctx->stream, g_ref->d_dev, g_sec->d_dev,
+                       op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
+                       (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
+                       pow_sec);
+  } else if (interp == S1IFG_INTERP_BILINEAR)
     hipLaunchKernelGGL(k_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, ctx->stream, g_ref->d_dev,
                        g_sec->d_dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
                        (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
