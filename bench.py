@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--config", default="c2", choices=["c1", "c2"])
     p.add_argument("--interp", default="sinc", choices=["bilinear", "sinc"])
     p.add_argument("--forms", default="plain,fused,batched")
+    p.add_argument("--tune-log", action="store_true", help="print every launch shape the autotuner times")
     p.add_argument("--no-iw", action="store_true", help="skip the 27-burst IW pair")
     p.add_argument("--iw-steps", type=int, default=5)
     p.add_argument("--no-adjoint", action="store_true", help="skip the C2 forward+backward")
@@ -169,7 +170,8 @@ def run_gpu(args):
                                  coh=coh)],
                       interp=args.interp, rl=rl, cl=cl, alpha=0.5, gold_mode=gold_mode,
                       coh_win=5, coh_fill="clamp")
-        shape, tms, _ = autotune.tune(b, reps=2)
+        tlog = (lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)) if args.tune_log else None
+        shape, tms, _ = autotune.tune(b, reps=2, log=tlog)
         print(f"[bench] batched launch shape {shape} ({tms:.3f} ms while tuning)",
               file=sys.stderr, flush=True)
         holder["batch"] = b

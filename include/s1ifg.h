@@ -303,6 +303,10 @@ typedef struct {
   int32_t stream_blocks_per_cu; /* deramp / overlap-add / coherence */
 } s1ifg_batch_opts;
 
+/* A *_blocks_per_cu value: one block per work item (a grid as large as the
+ * work) instead of persistent blocks; 0 selects the default. */
+#define S1IFG_BLOCK_PER_ITEM (-1)
+
 typedef struct s1ifg_batch s1ifg_batch;
 
 /* All bursts of an IW pair (or of several pairs of a stack) in one launch

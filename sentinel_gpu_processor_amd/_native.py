@@ -91,6 +91,9 @@ class BatchOpts(ctypes.Structure):
                 ("stream_blocks_per_cu", ctypes.c_int32)]
 
 
+# a *_blocks_per_cu value: one block per work item instead of persistent blocks
+BLOCK_PER_ITEM = -1
+
 LAUNCH_FIELDS = ("pair_block_x", "pair_blocks_per_cu", "gold_blocks_per_cu",
                  "stream_blocks_per_cu")
 

@@ -14,11 +14,13 @@ import time
 
 import torch
 
+from . import _native as N
+
 CANDIDATES = {
     "pair_block_x": (32, 64, 128),
-    "pair_blocks_per_cu": (3, 4, 6, 8, 16),
+    "pair_blocks_per_cu": (3, 4, 6, 8, 16, N.BLOCK_PER_ITEM),
     "gold_blocks_per_cu": (1, 2, 3, 4),
-    "stream_blocks_per_cu": (4, 8, 16, 32, 64),
+    "stream_blocks_per_cu": (4, 8, 16, 32, 64, N.BLOCK_PER_ITEM),
 }
 
 

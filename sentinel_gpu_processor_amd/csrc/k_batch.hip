@@ -10,6 +10,8 @@
 // own stream) and replayed: every kernel writes all of its outputs, the
 // work-queue counters are zeroed by a kernel (no memset), and the raw bursts
 // are never written.
+#include <algorithm>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -175,13 +177,26 @@ __global__ void kb_reset(unsigned long long *ctr, int n) {
 // pair (rl * cl <= kPairMaxPix).  Persistent blocks take tiles from a work
 // queue counter, so the tiles in flight on the chip stay together in the
 // bursts (cache reuse) whatever the grid size; the next tile is fetched
-// while the current one runs.
+// while the current one runs.  A grid with a block per tile runs one tile
+// per block in block order without the queue.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kb_pair_tiles(
     const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, int rl, int cl,
     unsigned long long *ctr) {
   __shared__ PairLds L;
   __shared__ long slot[2];
   const long total = off[n];
+  if (gridDim.x >= total) {
+    // one tile per block, no queue
+    const long t = blockIdx.x;
+    const int p = find_item(off, n, t);
+    const PairDev &d = *(const PairDev *)&P[p];
+    const long lt = t - off[p];
+    const int tiles_x = (d.nc_ml + kPairTile - 1) / kPairTile;
+    const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
+    pair_tile_sinc(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
+                   d.sec_d, rl, cl, ty, tx, d.nr_ml, d.nc_ml, d.ifg, d.pr, d.ps, L);
+    return;
+  }
   if (threadIdx.x == 0) slot[0] = (long)atomicAdd(ctr, 1ull);
   __syncthreads();
   int cur = 0, p = -1;
@@ -310,9 +325,9 @@ int fill_defaults(s1ifg_batch_opts *o) {
   const int bx = o->pair_block_x;
   if (bx != 16 && bx != 32 && bx != 64 && bx != 128 && bx != 256)
     return set_error(S1IFG_EINVAL, "pair_block_x %d (16, 32, 64, 128 or 256)", bx);
-  if (o->pair_blocks_per_cu < 1 || o->pair_blocks_per_cu > 64 || o->gold_blocks_per_cu < 1 ||
-      o->gold_blocks_per_cu > 64 || o->stream_blocks_per_cu < 1 || o->stream_blocks_per_cu > 64)
-    return set_error(S1IFG_EINVAL, "blocks per CU must be in [1, 64]");
+  for (int v : {o->pair_blocks_per_cu, o->gold_blocks_per_cu, o->stream_blocks_per_cu})
+    if (v != S1IFG_BLOCK_PER_ITEM && (v < 1 || v > 64))
+      return set_error(S1IFG_EINVAL, "blocks per CU must be in [1, 64] or S1IFG_BLOCK_PER_ITEM");
   return S1IFG_OK;
 }
 
@@ -341,9 +356,13 @@ void drop_graph(s1ifg_batch *b) {
   b->graph = nullptr;
 }
 
-unsigned grid_of(long items, long per_block, long cap) {
-  long want = (items + per_block - 1) / per_block;
-  return (unsigned)(want < cap ? want : cap);
+// Grid for `items` work items, `per_block` per block: capped at
+// blocks_per_cu blocks per CU (persistent blocks), or one block per
+// per_block items for S1IFG_BLOCK_PER_ITEM.
+unsigned grid_of(long items, long per_block, long ncu, int blocks_per_cu) {
+  const long want = (items + per_block - 1) / per_block;
+  const long cap = blocks_per_cu == S1IFG_BLOCK_PER_ITEM ? want : ncu * blocks_per_cu;
+  return (unsigned)std::min(std::min(want, cap), (long)INT_MAX);
 }
 
 // The launch sequence, enqueued on stream s.
@@ -358,18 +377,18 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
   hipLaunchKernelGGL(kb_reset, dim3(1), dim3(64), 0, s, b->d_ctr, (int)kNumStages);
   S1_LAUNCH_CHECK("kb_reset");
   if (long t = total(kStTables)) {
-    hipLaunchKernelGGL(kb_tables, dim3(grid_of(t, 256, ncu * o.stream_blocks_per_cu)), dim3(256),
+    hipLaunchKernelGGL(kb_tables, dim3(grid_of(t, 256, ncu, o.stream_blocks_per_cu)), dim3(256),
                        0, s, pairs, off[kStTables], n);
     S1_LAUNCH_CHECK("kb_tables");
   }
   if (long t = total(kStDeramp)) {
-    hipLaunchKernelGGL(kb_deramp, dim3(grid_of(t, 1, ncu * o.stream_blocks_per_cu)), dim3(256), 0,
+    hipLaunchKernelGGL(kb_deramp, dim3(grid_of(t, 1, ncu, o.stream_blocks_per_cu)), dim3(256), 0,
                        s, (const S1C ImgDev *)b->d_imgs, off[kStDeramp], (int)b->imgs.size());
     S1_LAUNCH_CHECK("kb_deramp");
   }
   if (long t = total(kStPair)) {
     dim3 blk(o.pair_block_x, 256 / o.pair_block_x);
-    dim3 grd(grid_of(t, 1, ncu * o.pair_blocks_per_cu));
+    dim3 grd(grid_of(t, 1, ncu, o.pair_blocks_per_cu));
     if (use_pair_tiles(o))
       hipLaunchKernelGGL(kb_pair_tiles, grd, dim3(256), 0, s, pairs, off[kStPair], n, o.rl,
                          o.cl, b->d_ctr + kStPair);
@@ -383,20 +402,20 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
   }
   if (long t = total(kStGold)) {
     if (o.gold_mode == S1IFG_GOLD_FP16_SPLIT)
-      hipLaunchKernelGGL(kb_gold_band, dim3(grid_of(t, kBandWaves, ncu * o.gold_blocks_per_cu)),
+      hipLaunchKernelGGL(kb_gold_band, dim3(grid_of(t, kBandWaves, ncu, o.gold_blocks_per_cu)),
                          dim3(64 * kBandWaves), 0, s, pairs, off[kStGold], n, o.alpha);
     else
-      hipLaunchKernelGGL(kb_gold_fp32, dim3(grid_of(t, kGoldGroups, ncu * 8 * o.gold_blocks_per_cu)),
+      hipLaunchKernelGGL(kb_gold_fp32, dim3(grid_of(t, kGoldGroups, ncu, o.gold_blocks_per_cu == S1IFG_BLOCK_PER_ITEM ? S1IFG_BLOCK_PER_ITEM : 8 * o.gold_blocks_per_cu)),
                          dim3(32 * kGoldGroups), 0, s, pairs, off[kStGold], n, o.alpha);
     S1_LAUNCH_CHECK("kb_gold");
   }
   if (long t = total(kStGather)) {
-    hipLaunchKernelGGL(kb_gather, dim3(grid_of(t, 1, ncu * o.stream_blocks_per_cu)), dim3(64, 4),
+    hipLaunchKernelGGL(kb_gather, dim3(grid_of(t, 1, ncu, o.stream_blocks_per_cu)), dim3(64, 4),
                        0, s, pairs, off[kStGather], n, o.gold_mode == S1IFG_GOLD_FP16_SPLIT);
     S1_LAUNCH_CHECK("kb_gather");
   }
   if (long t = total(kStCoh)) {
-    hipLaunchKernelGGL(kb_coherence, dim3(grid_of(t, 1, ncu * o.stream_blocks_per_cu)),
+    hipLaunchKernelGGL(kb_coherence, dim3(grid_of(t, 1, ncu, o.stream_blocks_per_cu)),
                        dim3(kCohTX, kCohTY), 0, s, pairs, off[kStCoh], n, o.coh_win,
                        o.coh_fill, o.coh_seed);
     S1_LAUNCH_CHECK("kb_coherence");

@@ -94,6 +94,7 @@ def test_batch_bit_identical_to_single_pairs(ctx, pairs, interp, gold_mode, win)
 
 
 def test_batch_launch_shapes_do_not_change_results(ctx, pairs):
+    from sentinel_gpu_processor_amd import _native as N
     from sentinel_gpu_processor_amd.batch import PairBatch
     want = [_single(ctx, p, "sinc", "fp16_split", 5) for p in pairs]
     b = PairBatch(ctx, pairs, interp="sinc", gold_mode="fp16_split", coh_win=5, seed=SEED)
@@ -101,7 +102,9 @@ def test_batch_launch_shapes_do_not_change_results(ctx, pairs):
                    "stream_blocks_per_cu": 1},
                   {"pair_block_x": 128, "pair_blocks_per_cu": 16, "gold_blocks_per_cu": 4,
                    "stream_blocks_per_cu": 32},
-                  {"pair_block_x": 256, "pair_blocks_per_cu": 2}):
+                  {"pair_block_x": 256, "pair_blocks_per_cu": 2},
+                  {"pair_blocks_per_cu": N.BLOCK_PER_ITEM, "gold_blocks_per_cu": N.BLOCK_PER_ITEM,
+                   "stream_blocks_per_cu": N.BLOCK_PER_ITEM}):
         b.launch = shape
         got = b.launch
         for k, v in shape.items():
