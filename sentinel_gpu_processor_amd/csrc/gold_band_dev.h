@@ -53,13 +53,9 @@ __device__ __forceinline__ void band_item(const mf::Consts &K, const float2 *__r
   __builtin_amdgcn_wave_barrier();
   int lo = ox0;
   const long rowbase = (long)pi * pj_n;
-  float2 nxt[16];
-  mf::load_patch(img, nrow, ncol, pj_n, rowbase + j0, h, n, nxt);
   for (int j = j0; j < j1; ++j) {
     float2 x[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) x[r] = nxt[r];
-    if (j + 1 < j1) mf::load_patch(img, nrow, ncol, pj_n, rowbase + j + 1, h, n, nxt);
+    mf::load_patch(img, nrow, ncol, pj_n, rowbase + j, h, n, x);
     float2 out[16];
     mf::filter_patch_regs(K, x, h, n, alpha, out);
     // add the patch (each lane owns distinct (row, column) entries)

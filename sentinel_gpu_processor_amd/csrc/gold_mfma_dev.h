@@ -4,7 +4,7 @@
 //
 // One wavefront filters one 32x32 patch entirely in registers:
 //   Y  = X F          (row transform)       Z^T = Y^T F   (column transform)
-//   S  = B1^T M B1 + (l2/l1) B2^T M B2     (7x7 circular smoothing of M=|Z|)
+//   S  = l1 B1^T M B1 + l2 B2^T M B2       (7x7 circular smoothing of M=|Z|)
 //   Zw = Z * (S/mean(S))^alpha where positive
 //   Y' = Zw conj(F)   z'^T = Y'^T conj(F)  (unnormalised inverse)
 // F is the 32-point DFT matrix; B_r are banded circulant matrices built from
@@ -75,12 +75,12 @@ __device__ __forceinline__ void to_half16(const float v[16], half8 hi[2]) {
 }
 
 struct CplxOp {  // constant complex B operand, split, per chunk
-  half8 rh[2], rl[2], ih[2], il[2], nih[2], nil[2];
+  half8 rh[2], rl[2], ih[2], il[2];
 };
 
 struct Consts {
-  CplxOp F, Fc;
-  half8 u1h[2], u1l[2], v1h[2], v1l[2], u2h[2], v2h[2];
+  CplxOp F;
+  half8 u1h[2], u1l[2], u2h[2];
 };
 
 __device__ __forceinline__ void split1(float x, _Float16 &h, _Float16 &l) {
@@ -102,65 +102,79 @@ __device__ __forceinline__ void build_consts(int h, int n, Consts &K) {
       split1(cs, a, b);
       K.F.rh[c][j] = a;
       K.F.rl[c][j] = b;
-      K.Fc.rh[c][j] = a;
-      K.Fc.rl[c][j] = b;
       split1(-sn, a, b);  // Im F = -sin
       K.F.ih[c][j] = a;
       K.F.il[c][j] = b;
-      K.F.nih[c][j] = -a;
-      K.F.nil[c][j] = -b;
-      K.Fc.ih[c][j] = -a;  // Im conj(F) = +sin
-      K.Fc.il[c][j] = -b;
-      K.Fc.nih[c][j] = a;
-      K.Fc.nil[c][j] = b;
       const int d = ((k - n + 16) & 31) - 16;  // circular offset
       const float b1 = (d >= -3 && d <= 3) ? kU1[d + 3] : 0.f;
       const float b2 = (d >= -3 && d <= 3) ? kU2[d + 3] : 0.f;
       split1(b1, a, b);
       K.u1h[c][j] = a;
       K.u1l[c][j] = b;
-      split1(kLam1 * b1, a, b);
-      K.v1h[c][j] = a;
-      K.v1l[c][j] = b;
       K.u2h[c][j] = (_Float16)b2;
-      K.v2h[c][j] = (_Float16)(kLam1 * b2);
     }
 }
 
-// C = A * B, A given by split real/imag parts, B constant.  Six independent
-// accumulator chains so that consecutive MFMAs do not wait on each other.
+// C = A * B (B = F) or A * conj(B) (CONJ), A given by split real/imag
+// parts, B constant.  The minus signs of the complex product go on a
+// negated copy of A's imaginary (B = F) or real (CONJ) part.  Four
+// accumulator chains: the hi*hi products of the real and imaginary parts,
+// and the hi*lo + lo*hi products of each (both carry the 2^11 scale of the
+// lo parts), interleaved so that consecutive MFMAs do not wait on each
+// other.
+template <bool CONJ>
 __device__ __forceinline__ void cgemm(const half8 arh[2], const half8 arl[2], const half8 aih[2],
                                       const half8 ail[2], const CplxOp &b, float cr[16],
                                       float ci[16]) {
-  f32x16 rm = {}, im = {}, rc1 = {}, ic1 = {}, rc2 = {}, ic2 = {};
+  f32x16 rm = {}, im = {}, rc = {}, ic = {};
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    rm = mfma(arh[c], b.rh[c], rm);
-    im = mfma(arh[c], b.ih[c], im);
-    rc1 = mfma(arh[c], b.rl[c], rc1);
-    ic1 = mfma(arh[c], b.il[c], ic1);
-    rc2 = mfma(arl[c], b.rh[c], rc2);
-    ic2 = mfma(arl[c], b.ih[c], ic2);
-    rm = mfma(aih[c], b.nih[c], rm);
-    im = mfma(aih[c], b.rh[c], im);
-    rc1 = mfma(aih[c], b.nil[c], rc1);
-    ic1 = mfma(aih[c], b.rl[c], ic1);
-    rc2 = mfma(ail[c], b.nih[c], rc2);
-    ic2 = mfma(ail[c], b.rh[c], ic2);
+    if (!CONJ) {
+      // Cr = Ar Fr - Ai Fi, Ci = Ar Fi + Ai Fr
+      const half8 nih = -aih[c], nil = -ail[c];
+      rm = mfma(arh[c], b.rh[c], rm);
+      im = mfma(arh[c], b.ih[c], im);
+      rc = mfma(arh[c], b.rl[c], rc);
+      ic = mfma(arh[c], b.il[c], ic);
+      rm = mfma(nih, b.ih[c], rm);
+      im = mfma(aih[c], b.rh[c], im);
+      rc = mfma(arl[c], b.rh[c], rc);
+      ic = mfma(arl[c], b.ih[c], ic);
+      rc = mfma(nih, b.il[c], rc);
+      ic = mfma(aih[c], b.rl[c], ic);
+      rc = mfma(nil, b.ih[c], rc);
+      ic = mfma(ail[c], b.rh[c], ic);
+    } else {
+      // Cr = Ar Fr + Ai Fi, Ci = Ai Fr - Ar Fi
+      const half8 nrh = -arh[c], nrl = -arl[c];
+      rm = mfma(arh[c], b.rh[c], rm);
+      im = mfma(nrh, b.ih[c], im);
+      rc = mfma(arh[c], b.rl[c], rc);
+      ic = mfma(nrh, b.il[c], ic);
+      rm = mfma(aih[c], b.ih[c], rm);
+      im = mfma(aih[c], b.rh[c], im);
+      rc = mfma(arl[c], b.rh[c], rc);
+      ic = mfma(nrl, b.ih[c], ic);
+      rc = mfma(aih[c], b.il[c], rc);
+      ic = mfma(aih[c], b.rl[c], ic);
+      rc = mfma(ail[c], b.ih[c], rc);
+      ic = mfma(ail[c], b.rh[c], ic);
+    }
   }
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    cr[r] = fmaf(rc1[r] + rc2[r], kLoInv, rm[r]);
-    ci[r] = fmaf(ic1[r] + ic2[r], kLoInv, im[r]);
+    cr[r] = fmaf(rc[r], kLoInv, rm[r]);
+    ci[r] = fmaf(ic[r], kLoInv, im[r]);
   }
 }
 
+template <bool CONJ>
 __device__ __forceinline__ void cgemm_v(const float xr[16], const float xi[16], const CplxOp &b,
                                         float yr[16], float yi[16]) {
   half8 arh[2], arl[2], aih[2], ail[2];
   split16(xr, arh, arl);
   split16(xi, aih, ail);
-  cgemm(arh, arl, aih, ail, b, yr, yi);
+  cgemm<CONJ>(arh, arl, aih, ail, b, yr, yi);
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -212,8 +226,8 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 
   }
   // forward transform
   float yr[16], yi[16], zr[16], zi[16];
-  cgemm_v(xr, xi, K.F, yr, yi);
-  cgemm_v(yr, yi, K.F, zr, zi);
+  cgemm_v<false>(xr, xi, K.F, yr, yi);
+  cgemm_v<false>(yr, yi, K.F, zr, zi);
   // smoothing of |Z|
   float m[16];
 #pragma unroll
@@ -228,6 +242,8 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 
     t2 = mfma(mh[c], K.u2h[c], t2);
     t1c = mfma(ml[c], K.u1h[c], t1c);
   }
+  // S = l1 (B1^T M B1 + (l2/l1) B2^T M B2): the second pass runs on B_r,
+  // l2/l1 scales the rank-2 term's data and l1 the sum
   float t1[16], t2s[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -240,16 +256,16 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 
   f32x16 sm = {}, sc = {}, sm2 = {};
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    sm = mfma(t1h[c], K.v1h[c], sm);
-    sc = mfma(t1h[c], K.v1l[c], sc);
-    sm2 = mfma(t2h[c], K.v2h[c], sm2);
-    sc = mfma(t1l[c], K.v1h[c], sc);
+    sm = mfma(t1h[c], K.u1h[c], sm);
+    sc = mfma(t1h[c], K.u1l[c], sc);
+    sm2 = mfma(t2h[c], K.u2h[c], sm2);
+    sc = mfma(t1l[c], K.u1h[c], sc);
   }
   float s[16];
   float lsum = 0.f;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    s[r] = fmaf(sc[r], kLoInv, sm[r] + sm2[r]);
+    s[r] = fmaf(sc[r], kLoInv, sm[r] + sm2[r]) * kLam1;
     lsum += s[r];
   }
   const float mean = wave_sum(lsum) * (1.0f / (kN * kN));
@@ -266,8 +282,8 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 
     }
   }
   // inverse transform
-  cgemm_v(zr, zi, K.Fc, yr, yi);
-  cgemm_v(yr, yi, K.Fc, zr, zi);
+  cgemm_v<true>(zr, zi, K.F, yr, yi);
+  cgemm_v<true>(yr, yi, K.F, zr, zi);
   // weighted output (goldstein.cu:257-267), undo the patch scale
   const float wy = (float)(n < kN / 2 ? n : kN - 1 - n);
   const float norm = 1.0f / (kN * kN * scale);

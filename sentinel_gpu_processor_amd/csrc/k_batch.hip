@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void kb_pair(const S1C PairDev *__restrict__ P
 
 // Goldstein FP16 split: (band, chunk) items of every pair on persistent
 // wavefronts, overlap-add fused (gold_band_dev.h).
-__global__ __launch_bounds__(64 * kBandWaves) void kb_gold_band(
+__global__ __launch_bounds__(64 * kBandWaves) __attribute__((amdgpu_waves_per_eu(2))) void kb_gold_band(
     const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, float alpha) {
   __shared__ BandLds L;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

@@ -7,7 +7,7 @@ namespace s1ifg {
 namespace {
 
 // Persistent wavefronts over the (band, chunk) items (no block barriers).
-__global__ __launch_bounds__(64 * kBandWaves) void k_gold_band(
+__global__ __launch_bounds__(64 * kBandWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_gold_band(
     const float2 *__restrict__ img, int nrow, int ncol, int pi_n, int pj_n, float alpha,
     float2 *__restrict__ partial) {
   __shared__ BandLds L;
