@@ -20,9 +20,17 @@
 // kappa(c,h,j) = 16c + 8(j/4) + 4h + (j%4), and the constant B operands are
 // generated in that order.
 //
-// Split.  x = hi + lo/2^11 with hi = fp16(x), lo = fp16((x - hi) 2^11);
-// products keep hi*hi + (hi*lo + lo*hi)/2^11 (lo*lo dropped), accumulated
-// in FP32.  Each patch is scaled by a power of two to max |x| < 1 first.
+// Split.  Every MFMA A operand v is split v = hi + lo with hi = fp16(v),
+// lo = fp16(v - hi); the B constants are split the same way.  A product
+// keeps hi*hi + hi*lo + lo*hi (lo*lo, ~2^-22 relative, dropped); all three
+// carry the same scale and go into one FP32 accumulator.  Operand ranges:
+// the patch is scaled by a power of two to max component in [4, 8); a
+// 32-point transform grows the maximum by at most 32 sqrt(2), so
+// |Y| < 2^8.5, |Z| < 2^14 and the smoothing passes stay below 1.9 * 2^14.5
+// (all within FP16); the weighted spectrum is rescaled like the patch
+// before the inverse transform, and both scales are undone exactly at the
+// end.  Values below 2^-11 of their operand's maximum have subnormal lo
+// parts: their absolute error stays below 2^-27 of that maximum.
 #pragma once
 
 #include "gold_dev.h"
@@ -35,8 +43,7 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kN = 32;
-constexpr float kLoScale = 2048.0f;
-constexpr float kLoInv = 1.0f / 2048.0f;
+constexpr int kNormExp = 3;  // max component of a normalised operand in [4, 8)
 // leading eigen-pairs of the 7x7 smoothing table (even eigenvectors)
 constexpr float kLam1 = 3.698581918e-01f;
 constexpr float kLam2 = -5.889286314e-05f;
@@ -55,23 +62,27 @@ __device__ __forceinline__ int kappa(int c, int h, int j) {
   return 16 * c + 8 * (j >> 2) + 4 * h + (j & 3);
 }
 
+// hi + lo split of v (|v| < 2^16).  lo is one mixed-precision FMA
+// v * 1 - hi per value (the 1 is hidden from constant folding so that the
+// FMA is not rewritten as a subtraction with separate conversions).
 __device__ __forceinline__ void split16(const float v[16], half8 hi[2], half8 lo[2]) {
+  float one = 1.0f;
+  asm("" : "+v"(one));
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      float x = v[8 * c + j];
-      _Float16 xh = (_Float16)x;
-      hi[c][j] = xh;
-      lo[c][j] = (_Float16)((x - (float)xh) * kLoScale);
+      const _Float16 h = (_Float16)v[8 * c + j];
+      hi[c][j] = h;
+      lo[c][j] = (_Float16)fmaf(v[8 * c + j], one, -(float)h);
     }
 }
 
-__device__ __forceinline__ void to_half16(const float v[16], half8 hi[2]) {
+__device__ __forceinline__ void to_half16(const float v[16], float s, half8 hi[2]) {
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) hi[c][j] = (_Float16)v[8 * c + j];
+    for (int j = 0; j < 8; ++j) hi[c][j] = (_Float16)(v[8 * c + j] * s);
 }
 
 struct CplxOp {  // constant complex B operand, split, per chunk
@@ -85,7 +96,7 @@ struct Consts {
 
 __device__ __forceinline__ void split1(float x, _Float16 &h, _Float16 &l) {
   h = (_Float16)x;
-  l = (_Float16)((x - (float)h) * kLoScale);
+  l = (_Float16)(x - (float)h);
 }
 
 // Constant B operands of lane (h, n) in kappa order.
@@ -117,57 +128,56 @@ __device__ __forceinline__ void build_consts(int h, int n, Consts &K) {
 
 // C = A * B (B = F) or A * conj(B) (CONJ), A given by split real/imag
 // parts, B constant.  The minus signs of the complex product go on a
-// negated copy of A's imaginary (B = F) or real (CONJ) part.  Four
-// accumulator chains: the hi*hi products of the real and imaginary parts,
-// and the hi*lo + lo*hi products of each (both carry the 2^11 scale of the
-// lo parts), interleaved so that consecutive MFMAs do not wait on each
-// other.
+// negated copy of A's imaginary (B = F) or real (CONJ) part.  One
+// accumulator for the real and one for the imaginary part, alternating, the
+// hi*lo terms of a K chunk before its hi*hi terms.
 template <bool CONJ>
 __device__ __forceinline__ void cgemm(const half8 arh[2], const half8 arl[2], const half8 aih[2],
                                       const half8 ail[2], const CplxOp &b, float cr[16],
                                       float ci[16]) {
-  f32x16 rm = {}, im = {}, rc = {}, ic = {};
+  f32x16 re = {}, im = {};
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     if (!CONJ) {
       // Cr = Ar Fr - Ai Fi, Ci = Ar Fi + Ai Fr
       const half8 nih = -aih[c], nil = -ail[c];
-      rm = mfma(arh[c], b.rh[c], rm);
+      re = mfma(arl[c], b.rh[c], re);
+      im = mfma(arl[c], b.ih[c], im);
+      re = mfma(arh[c], b.rl[c], re);
+      im = mfma(arh[c], b.il[c], im);
+      re = mfma(nil, b.ih[c], re);
+      im = mfma(ail[c], b.rh[c], im);
+      re = mfma(nih, b.il[c], re);
+      im = mfma(aih[c], b.rl[c], im);
+      re = mfma(arh[c], b.rh[c], re);
       im = mfma(arh[c], b.ih[c], im);
-      rc = mfma(arh[c], b.rl[c], rc);
-      ic = mfma(arh[c], b.il[c], ic);
-      rm = mfma(nih, b.ih[c], rm);
+      re = mfma(nih, b.ih[c], re);
       im = mfma(aih[c], b.rh[c], im);
-      rc = mfma(arl[c], b.rh[c], rc);
-      ic = mfma(arl[c], b.ih[c], ic);
-      rc = mfma(nih, b.il[c], rc);
-      ic = mfma(aih[c], b.rl[c], ic);
-      rc = mfma(nil, b.ih[c], rc);
-      ic = mfma(ail[c], b.rh[c], ic);
     } else {
       // Cr = Ar Fr + Ai Fi, Ci = Ai Fr - Ar Fi
       const half8 nrh = -arh[c], nrl = -arl[c];
-      rm = mfma(arh[c], b.rh[c], rm);
-      im = mfma(nrh, b.ih[c], im);
-      rc = mfma(arh[c], b.rl[c], rc);
-      ic = mfma(nrh, b.il[c], ic);
-      rm = mfma(aih[c], b.ih[c], rm);
+      re = mfma(arl[c], b.rh[c], re);
+      im = mfma(ail[c], b.rh[c], im);
+      re = mfma(arh[c], b.rl[c], re);
+      im = mfma(aih[c], b.rl[c], im);
+      re = mfma(ail[c], b.ih[c], re);
+      im = mfma(nrl, b.ih[c], im);
+      re = mfma(aih[c], b.il[c], re);
+      im = mfma(nrh, b.il[c], im);
+      re = mfma(arh[c], b.rh[c], re);
       im = mfma(aih[c], b.rh[c], im);
-      rc = mfma(arl[c], b.rh[c], rc);
-      ic = mfma(nrl, b.ih[c], ic);
-      rc = mfma(aih[c], b.il[c], rc);
-      ic = mfma(aih[c], b.rl[c], ic);
-      rc = mfma(ail[c], b.ih[c], rc);
-      ic = mfma(ail[c], b.rh[c], ic);
+      re = mfma(aih[c], b.ih[c], re);
+      im = mfma(nrh, b.ih[c], im);
     }
   }
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    cr[r] = fmaf(rc[r], kLoInv, rm[r]);
-    ci[r] = fmaf(ic[r], kLoInv, im[r]);
+    cr[r] = re[r];
+    ci[r] = im[r];
   }
 }
 
+// Y = X B or X conj(B) for X = xr + i xi.
 template <bool CONJ>
 __device__ __forceinline__ void cgemm_v(const float xr[16], const float xi[16], const CplxOp &b,
                                         float yr[16], float yi[16]) {
@@ -210,21 +220,18 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 
   float xr[16], xi[16];
   float mx = 0.f;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    xr[r] = x[r].x;
-    xi[r] = x[r].y;
-    mx = fmaxf(mx, fmaxf(fabsf(xr[r]), fabsf(xi[r])));
-  }
+  for (int r = 0; r < 16; ++r) mx = fmaxf(mx, fmaxf(fabsf(x[r].x), fabsf(x[r].y)));
   mx = wave_max(mx);
   int e = 0;
-  if (mx > 0.f) frexpf(mx, &e);
-  const float scale = ldexpf(1.0f, -e);  // max |x| * scale in [0.5, 1)
+  if (mx > 0.f) frexpf(mx, &e);  // max |x| in [2^(e-1), 2^e)
+  e = max(e, -123);               // keeps s0 = 2^(kNormExp - e) finite
+  const float s0 = ldexpf(1.0f, kNormExp - e);
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    xr[r] *= scale;
-    xi[r] *= scale;
+    xr[r] = x[r].x * s0;
+    xi[r] = x[r].y * s0;
   }
-  // forward transform
+  // forward transform: Z s0
   float yr[16], yi[16], zr[16], zi[16];
   cgemm_v<false>(xr, xi, K.F, yr, yi);
   cgemm_v<false>(yr, yi, K.F, zr, zi);
@@ -234,59 +241,72 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 
   for (int r = 0; r < 16; ++r) m[r] = sqrtf(zr[r] * zr[r] + zi[r] * zi[r]);
   half8 mh[2], ml[2];
   split16(m, mh, ml);
-  f32x16 t1m = {}, t1c = {}, t2 = {};
+  f32x16 t1 = {}, t2 = {};
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    t1m = mfma(mh[c], K.u1h[c], t1m);
-    t1c = mfma(mh[c], K.u1l[c], t1c);
+    t1 = mfma(mh[c], K.u1l[c], t1);
     t2 = mfma(mh[c], K.u2h[c], t2);
-    t1c = mfma(ml[c], K.u1h[c], t1c);
+    t1 = mfma(ml[c], K.u1h[c], t1);
+    t1 = mfma(mh[c], K.u1h[c], t1);
   }
   // S = l1 (B1^T M B1 + (l2/l1) B2^T M B2): the second pass runs on B_r,
   // l2/l1 scales the rank-2 term's data and l1 the sum
-  float t1[16], t2s[16];
+  float t1v[16], t2v[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    t1[r] = fmaf(t1c[r], kLoInv, t1m[r]);
-    t2s[r] = t2[r] * (kLam2 / kLam1);
+    t1v[r] = t1[r];
+    t2v[r] = t2[r];
   }
   half8 t1h[2], t1l[2], t2h[2];
-  split16(t1, t1h, t1l);
-  to_half16(t2s, t2h);
-  f32x16 sm = {}, sc = {}, sm2 = {};
+  split16(t1v, t1h, t1l);
+  to_half16(t2v, kLam2 / kLam1, t2h);
+  f32x16 sm = {};
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
+    sm = mfma(t2h[c], K.u2h[c], sm);
+    sm = mfma(t1h[c], K.u1l[c], sm);
+    sm = mfma(t1l[c], K.u1h[c], sm);
     sm = mfma(t1h[c], K.u1h[c], sm);
-    sc = mfma(t1h[c], K.u1l[c], sc);
-    sm2 = mfma(t2h[c], K.u2h[c], sm2);
-    sc = mfma(t1l[c], K.u1h[c], sc);
   }
   float s[16];
   float lsum = 0.f;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    s[r] = fmaf(sc[r], kLoInv, sm[r] + sm2[r]) * kLam1;
+    s[r] = sm[r] * kLam1;
     lsum += s[r];
   }
   const float mean = wave_sum(lsum) * (1.0f / (kN * kN));
-  // goldstein.cu:170-175 on the unscaled data: mean > 1e-8
-  const bool ok = mean > 1e-8f * scale;
+  // goldstein.cu:170-175 on the unscaled data: mean > 1e-8 (S carries s0)
+  const bool ok = mean > 1e-8f * s0;
   const float inv = ok ? 1.0f / mean : 0.f;
+  float mw = 0.f;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     float sn = s[r] * inv;
-    if (sn > 0.f) {  // powf(sn, alpha) > 0 only here
+    if (sn > 0.f) {  // powf(sn, alpha) > 0 only here; Z is kept elsewhere
       float w = __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(sn));
       zr[r] *= w;
       zi[r] *= w;
     }
+    mw = fmaxf(mw, fmaxf(fabsf(zr[r]), fabsf(zi[r])));
   }
-  // inverse transform
+  mw = wave_max(mw);
+  int e2 = 0;
+  if (mw > 0.f) frexpf(mw, &e2);
+  e2 = max(e2, -123);
+  const float s2 = ldexpf(1.0f, kNormExp - e2);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    zr[r] *= s2;
+    zi[r] *= s2;
+  }
+  // inverse transform of the rescaled spectrum: z' n^2 s0 s2
   cgemm_v<true>(zr, zi, K.F, yr, yi);
   cgemm_v<true>(yr, yi, K.F, zr, zi);
   // weighted output (goldstein.cu:257-267), undo the patch scale
   const float wy = (float)(n < kN / 2 ? n : kN - 1 - n);
-  const float norm = 1.0f / (kN * kN * scale);
+  // z' / n^2 = result / (s0 s2 n^2) = result 2^(e + e2 - 2 kNormExp - 10)
+  const float norm = ldexpf(1.0f, e + e2 - 2 * kNormExp - 10);
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
