@@ -33,30 +33,33 @@ struct CohTile {
   float sx[kCohH][kCohW], sy[kCohH][kCohW], s1[kCohH][kCohW], s2[kCohH][kCohW];
 };
 
-// Windowed coherence of one 64 x 4 output tile (block 64 x 4): the tile and
-// its halo are staged in LDS; sums over the window rows outer, columns
-// inner, zero outside the image.  Contains block barriers.
-__device__ __forceinline__ void coherence_tile(const float2 *__restrict__ ifg,
-                                               const float *__restrict__ p1,
-                                               const float *__restrict__ p2, int nr, int nc,
-                                               int win, int fill, uint32_t seed,
-                                               float *__restrict__ coh, int tile_x, int tile_y,
-                                               CohTile &L) {
+// Window sums of four channels (a complex, b, c) over a win x win window
+// centred on each pixel of one 64 x 4 output tile (block 64 x 4): the tile
+// and its halo are staged in LDS; sums over the window rows outer, columns
+// inner, zero outside the image.  Returns whether the thread's pixel
+// (r, c) is inside the image.  Contains a block barrier; callers pass
+// another before L is reused.
+__device__ __forceinline__ bool window_sums_tile(const float2 *__restrict__ za,
+                                                 const float *__restrict__ zb,
+                                                 const float *__restrict__ zc, int nr, int nc,
+                                                 int win, int tile_x, int tile_y, CohTile &L,
+                                                 int &r, int &c, float &ax, float &ay, float &a1,
+                                                 float &a2) {
   const int h = win / 2;
   const int tw = kCohTX + 2 * h, th = kCohTY + 2 * h;
   const int c0 = tile_x * kCohTX - h, r0 = tile_y * kCohTY - h;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   for (int i = tid; i < tw * th; i += blockDim.x * blockDim.y) {
     int ty = i / tw, tx = i - ty * tw;
-    int r = r0 + ty, c = c0 + tx;
+    int rr = r0 + ty, cc = c0 + tx;
     float a = 0.f, b = 0.f, u = 0.f, v = 0.f;
-    if (r >= 0 && r < nr && c >= 0 && c < nc) {
-      size_t o = (size_t)r * nc + c;
-      float2 z = ifg[o];
+    if (rr >= 0 && rr < nr && cc >= 0 && cc < nc) {
+      size_t o = (size_t)rr * nc + cc;
+      float2 z = za[o];
       a = z.x;
       b = z.y;
-      u = p1[o];
-      v = p2[o];
+      u = zb[o];
+      v = zc[o];
     }
     L.sx[ty][tx] = a;
     L.sy[ty][tx] = b;
@@ -64,18 +67,32 @@ __device__ __forceinline__ void coherence_tile(const float2 *__restrict__ ifg,
     L.s2[ty][tx] = v;
   }
   __syncthreads();
-  const int c = tile_x * kCohTX + threadIdx.x;
-  const int r = tile_y * kCohTY + threadIdx.y;
-  if (r < nr && c < nc) {
-    float ax = 0.f, ay = 0.f, a1 = 0.f, a2 = 0.f;
-    for (int i = 0; i < win; ++i)
-      for (int j = 0; j < win; ++j) {
-        int ty = threadIdx.y + i, tx = threadIdx.x + j;
-        ax += L.sx[ty][tx];
-        ay += L.sy[ty][tx];
-        a1 += L.s1[ty][tx];
-        a2 += L.s2[ty][tx];
-      }
+  c = tile_x * kCohTX + threadIdx.x;
+  r = tile_y * kCohTY + threadIdx.y;
+  ax = ay = a1 = a2 = 0.f;
+  if (r >= nr || c >= nc) return false;
+  for (int i = 0; i < win; ++i)
+    for (int j = 0; j < win; ++j) {
+      int ty = threadIdx.y + i, tx = threadIdx.x + j;
+      ax += L.sx[ty][tx];
+      ay += L.sy[ty][tx];
+      a1 += L.s1[ty][tx];
+      a2 += L.s2[ty][tx];
+    }
+  return true;
+}
+
+// Windowed coherence of one 64 x 4 output tile (block 64 x 4).  Contains
+// block barriers.
+__device__ __forceinline__ void coherence_tile(const float2 *__restrict__ ifg,
+                                               const float *__restrict__ p1,
+                                               const float *__restrict__ p2, int nr, int nc,
+                                               int win, int fill, uint32_t seed,
+                                               float *__restrict__ coh, int tile_x, int tile_y,
+                                               CohTile &L) {
+  int r, c;
+  float ax, ay, a1, a2;
+  if (window_sums_tile(ifg, p1, p2, nr, nc, win, tile_x, tile_y, L, r, c, ax, ay, a1, a2)) {
     float den = sqrtf(a1 * a2);
     float g = den > 0.f ? sqrtf(ax * ax + ay * ay) / den : 0.f;
     size_t o = (size_t)r * nc + c;

@@ -21,6 +21,8 @@
 // dgamma/dA = A / (|A| sqrt(B C)), dgamma/dB = -gamma / (2 B), and the
 // window sums are transposed by the same (clipped, symmetric) window sums.
 // Pixels whose coherence was filled (gamma > 1) or is 0 get no gradient.
+#include <climits>
+
 #include "geom_dev.h"
 #include "ifg_dev.h"
 
@@ -78,10 +80,14 @@ struct BurstAdj {
   bool valid;
 };
 
+// Geometry part of one burst at one target pixel: sampling position
+// (rg, az), the phase rotation (sp, cp) and the chain partials of o (all
+// but the interpolated values).  Returns validity.
 template <int INTERP>
-__device__ __forceinline__ void burst_adj(const GeomDev &g, const OffPoly &op, const double X[3],
-                                          const double nrm[3], const float2 *__restrict__ z,
-                                          BurstAdj &o) {
+__device__ __forceinline__ bool burst_geom_adj(const GeomDev &g, const OffPoly &op,
+                                               const double X[3], const double nrm[3],
+                                               double &rg, double &az, float &sp, float &cp,
+                                               BurstAdj &o) {
   double t, dr[3], v[3], a[3];
   zero_doppler(g, X, t, dr, v, a);
   double rng = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
@@ -93,17 +99,20 @@ __device__ __forceinline__ void burst_adj(const GeomDev &g, const OffPoly &op, c
     drg = op.p[0] + op.p[1] * xs + op.p[2] * ys;
     daz = op.p[3] + op.p[4] * xs + op.p[5] * ys;
   }
-  double rg = rgoff + drg, az = azoff + daz;
+  rg = rgoff + drg;
+  az = azoff + daz;
   o.xs = xs;
   o.ys = ys;
   if (INTERP == S1IFG_INTERP_BILINEAR)
     o.valid = rg > g.fvs && rg < g.lvs - 1 && az >= g.flv + 5 && az < g.llv - 5;
   else
     o.valid = rg >= g.fvs + 3 && rg < g.lvs - 4 && az >= g.flv + 5 && az < g.llv - 5;
+  o.z = o.dzr = o.dza = make_float2(0.f, 0.f);
   if (!o.valid) {
-    o.z = o.dzr = o.dza = make_float2(0.f, 0.f);
     o.R_rp = o.R_eta = o.dt_dh = o.drng_dh = 0.0;
-    return;
+    sp = 0.f;
+    cp = 1.f;
+    return false;
   }
   // reramp and its partials (FP64)
   const double c2 = 2.0 / kSol;
@@ -124,11 +133,62 @@ __device__ __forceinline__ void burst_adj(const GeomDev &g, const OffPoly &op, c
   o.R_rp = -kPi * (dkt * ed * ed - 2.0 * kt * ed * detaref);
   o.R_eta = -2.0 * kPi * kt * ed;
   float ph = reduce_phase(4.0 * kPi / g.wvl * rng + kPi * kt * ed * ed);
-  float sp, cp;
   sincosf(ph, &sp, &cp);
-  // interpolation and its derivatives at the sampling position
+  // height partials
+  double vn = v[0] * nrm[0] + v[1] * nrm[1] + v[2] * nrm[2];
+  double den = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] - (dr[0] * a[0] + dr[1] * a[1] + dr[2] * a[2]);
+  o.dt_dh = vn * rcp64(den);
+  o.drng_dh = (dr[0] * nrm[0] + dr[1] * nrm[1] + dr[2] * nrm[2]) * rcp64(rng);
+  return true;
+}
+
+// Rotate the interpolated value and its derivatives by (cp, sp) into o.
+__device__ __forceinline__ void set_rotated(float vx, float vy, float rx, float ry, float ax,
+                                            float ay, float sp, float cp, BurstAdj &o) {
+  o.z = make_float2(vx * cp - vy * sp, vx * sp + vy * cp);
+  o.dzr = make_float2(rx * cp - ry * sp, rx * sp + ry * cp);
+  o.dza = make_float2(ax * cp - ay * sp, ax * sp + ay * cp);
+}
+
+// 8 x 8 sinc value and its range / azimuth derivatives at fractional
+// offsets (fr, fa) from the window whose top-left tap is base (row stride
+// ld), rotated into o.
+__device__ __forceinline__ void sinc_adj(const float2 *__restrict__ base, long ld, float fr,
+                                         float fa, float sp, float cp, BurstAdj &o) {
+  float wr[8], dwr[8], wa[8], dwa[8];
+  sinc_weights_d(fr, wr, dwr);
+  sinc_weights_d(fa, wa, dwa);
+  float vx = 0.f, vy = 0.f, rx = 0.f, ry = 0.f, ax = 0.f, ay = 0.f;
+#pragma unroll 2
+  for (int ia = 0; ia < 8; ++ia) {
+    const float2 *row = base + ia * ld;
+    float sx = 0.f, sy = 0.f, dx = 0.f, dy = 0.f;
+#pragma unroll
+    for (int ir = 0; ir < 8; ++ir) {
+      float2 q = row[ir];
+      sx = fmaf(wr[ir], q.x, sx);
+      sy = fmaf(wr[ir], q.y, sy);
+      dx = fmaf(dwr[ir], q.x, dx);
+      dy = fmaf(dwr[ir], q.y, dy);
+    }
+    vx = fmaf(wa[ia], sx, vx);
+    vy = fmaf(wa[ia], sy, vy);
+    rx = fmaf(wa[ia], dx, rx);
+    ry = fmaf(wa[ia], dy, ry);
+    ax = fmaf(dwa[ia], sx, ax);
+    ay = fmaf(dwa[ia], sy, ay);
+  }
+  set_rotated(vx, vy, rx, ry, ax, ay, sp, cp, o);
+}
+
+template <int INTERP>
+__device__ __forceinline__ void burst_adj(const GeomDev &g, const OffPoly &op, const double X[3],
+                                          const double nrm[3], const float2 *__restrict__ z,
+                                          BurstAdj &o) {
+  double rg, az;
+  float sp, cp;
+  if (!burst_geom_adj<INTERP>(g, op, X, nrm, rg, az, sp, cp, o)) return;
   const int N = g.samples;
-  float vx, vy, rx_, ry_, ax_, ay_;
   if (INTERP == S1IFG_INTERP_BILINEAR) {
     int intr = (int)rg, inta = (int)az;
     float fr = (float)(rg - intr), fa = (float)(az - inta);
@@ -136,46 +196,14 @@ __device__ __forceinline__ void burst_adj(const GeomDev &g, const OffPoly &op, c
     float2 b1 = z[i1], b2 = z[i1 + 1], b3 = z[i1 + N], b4 = z[i1 + N + 1];
     float c1x = b1.x + (b2.x - b1.x) * fr, c1y = b1.y + (b2.y - b1.y) * fr;
     float c2x = b3.x + (b4.x - b3.x) * fr, c2y = b3.y + (b4.y - b3.y) * fr;
-    vx = c1x + (c2x - c1x) * fa;
-    vy = c1y + (c2y - c1y) * fa;
-    rx_ = (b2.x - b1.x) * (1.f - fa) + (b4.x - b3.x) * fa;
-    ry_ = (b2.y - b1.y) * (1.f - fa) + (b4.y - b3.y) * fa;
-    ax_ = c2x - c1x;
-    ay_ = c2y - c1y;
+    set_rotated(c1x + (c2x - c1x) * fa, c1y + (c2y - c1y) * fa,
+                (b2.x - b1.x) * (1.f - fa) + (b4.x - b3.x) * fa,
+                (b2.y - b1.y) * (1.f - fa) + (b4.y - b3.y) * fa, c2x - c1x, c2y - c1y, sp, cp, o);
   } else {
     int intr = (int)floor(rg), inta = (int)floor(az);
-    float wr[8], dwr[8], wa[8], dwa[8];
-    sinc_weights_d((float)(rg - intr), wr, dwr);
-    sinc_weights_d((float)(az - inta), wa, dwa);
-    vx = vy = rx_ = ry_ = ax_ = ay_ = 0.f;
-    const float2 *base = z + (size_t)(inta - 3) * N + (intr - 3);
-    for (int ia = 0; ia < 8; ++ia) {
-      const float2 *row = base + (size_t)ia * N;
-      float sx = 0.f, sy = 0.f, dx = 0.f, dy = 0.f;
-#pragma unroll
-      for (int ir = 0; ir < 8; ++ir) {
-        float2 q = row[ir];
-        sx = fmaf(wr[ir], q.x, sx);
-        sy = fmaf(wr[ir], q.y, sy);
-        dx = fmaf(dwr[ir], q.x, dx);
-        dy = fmaf(dwr[ir], q.y, dy);
-      }
-      vx = fmaf(wa[ia], sx, vx);
-      vy = fmaf(wa[ia], sy, vy);
-      rx_ = fmaf(wa[ia], dx, rx_);
-      ry_ = fmaf(wa[ia], dy, ry_);
-      ax_ = fmaf(dwa[ia], sx, ax_);
-      ay_ = fmaf(dwa[ia], sy, ay_);
-    }
+    sinc_adj(z + (size_t)(inta - 3) * N + (intr - 3), N, (float)(rg - intr), (float)(az - inta),
+             sp, cp, o);
   }
-  o.z = make_float2(vx * cp - vy * sp, vx * sp + vy * cp);
-  o.dzr = make_float2(rx_ * cp - ry_ * sp, rx_ * sp + ry_ * cp);
-  o.dza = make_float2(ax_ * cp - ay_ * sp, ax_ * sp + ay_ * cp);
-  // height partials
-  double vn = v[0] * nrm[0] + v[1] * nrm[1] + v[2] * nrm[2];
-  double den = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] - (dr[0] * a[0] + dr[1] * a[1] + dr[2] * a[2]);
-  o.dt_dh = vn * rcp64(den);
-  o.drng_dh = (dr[0] * nrm[0] + dr[1] * nrm[1] + dr[2] * nrm[2]) * rcp64(rng);
 }
 
 // Chain from dL/dout of one burst to the offset-polynomial partials (added to
@@ -269,13 +297,172 @@ __global__ __launch_bounds__(kAdjBlock) void k_pair_bwd(
   if (threadIdx.x < 6) partial[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Sum of the per-block partials in block order, one thread per coefficient.
-__global__ void k_reduce6(const double *__restrict__ partial, int nblocks, double *out) {
-  const int k = threadIdx.x;
-  if (k >= 6) return;
-  double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * 6 + k];
-  out[k] = s;
+// Sinc adjoint on tiles of kBwdTile x kBwdTile target pixels (one per
+// thread), the burst windows staged in LDS as in the forward tile kernel:
+// the geometry and chain partials of a pixel stay in registers while the
+// block stages the bounding box of its taps; a window larger than kBwdWin
+// samples is read from global memory instead (same arithmetic).
+constexpr int kBwdTile = 16;
+constexpr int kBwdWin = 1664;
+
+struct BwdLds {
+  float2 win[2][kBwdWin];
+  int box[2][4];
+  double red[kBwdTile * kBwdTile / 64][6];
+};
+
+__device__ __forceinline__ int wmin_i(int v) {
+  for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wmax_i(int v) {
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+__global__ __launch_bounds__(kBwdTile * kBwdTile) void k_pair_bwd_tiles(
+    const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr,
+    DemDev dem, const RowTab *rows, const ColTab *cols, const float *dh,
+    const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl, int cl, int nr_ml,
+    int nc_ml, const float2 *__restrict__ g_ifg, const float *__restrict__ g_pr,
+    const float *__restrict__ g_ps, double *__restrict__ partial, float *__restrict__ g_dh) {
+  __shared__ BwdLds L;
+  const GeomDev *G[2] = {gpr, gps};
+  const float2 *Z[2] = {zr, zs};
+  const OffPoly none = {{0, 0, 0, 0, 0, 0}, 0};
+  const int tid = threadIdx.x;
+  const int r = blockIdx.y * kBwdTile + tid / kBwdTile;
+  const int c = blockIdx.x * kBwdTile + (tid & (kBwdTile - 1));
+  const bool active = r < nr_ml * rl && c < nc_ml * cl;
+  if (tid < 8) L.box[tid >> 2][tid & 3] = (tid & 3) < 2 ? INT_MAX : INT_MIN;
+  BurstAdj b[2];
+  int ir[2] = {0, 0}, ia[2] = {0, 0};
+  float fr[2] = {0.f, 0.f}, fa[2] = {0.f, 0.f}, sp[2] = {0.f, 0.f}, cp[2] = {1.f, 1.f};
+  int lo_r[2] = {INT_MAX, INT_MAX}, lo_a[2] = {INT_MAX, INT_MAX};
+  int hi_r[2] = {INT_MIN, INT_MIN}, hi_a[2] = {INT_MIN, INT_MIN};
+  b[0].valid = b[1].valid = false;
+  if (active) {
+    double X[3];
+    pixel_xyz(gr, dem, rows, cols, dh, r, c, X);
+    const RowTab rt = rows[r];
+    const ColTab ct = cols[c];
+    const double nrm[3] = {rt.coslat * ct.coslon, rt.coslat * ct.sinlon, rt.sinlat};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      double rg, az;
+      if (burst_geom_adj<S1IFG_INTERP_SINC>(*G[k], k ? ops : none, X, nrm, rg, az, sp[k], cp[k],
+                                            b[k])) {
+        ir[k] = (int)floor(rg);
+        ia[k] = (int)floor(az);
+        fr[k] = (float)(rg - ir[k]);
+        fa[k] = (float)(az - ia[k]);
+        lo_r[k] = hi_r[k] = ir[k];
+        lo_a[k] = hi_a[k] = ia[k];
+      }
+    }
+  }
+  __syncthreads();  // box initialised
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int v0 = wmin_i(lo_r[k]), v1 = wmin_i(lo_a[k]);
+    const int v2 = wmax_i(hi_r[k]), v3 = wmax_i(hi_a[k]);
+    if ((tid & 63) == 0) {
+      atomicMin(&L.box[k][0], v0);
+      atomicMin(&L.box[k][1], v1);
+      atomicMax(&L.box[k][2], v2);
+      atomicMax(&L.box[k][3], v3);
+    }
+  }
+  __syncthreads();
+  int wc[2], wr0[2], wa0[2];
+  bool staged[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int mr = L.box[k][0], ma = L.box[k][1], xr = L.box[k][2], xa = L.box[k][3];
+    const bool any = mr <= xr;
+    const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
+    staged[k] = any && W * H <= kBwdWin;
+    wc[k] = (int)W;
+    wr0[k] = mr - 3;
+    wa0[k] = ma - 3;
+    if (staged[k]) {
+      const int ns = G[k]->samples;
+      for (int row = tid >> 6; row < H; row += kBwdTile * kBwdTile / 64)
+        for (int col = tid & 63; col < W; col += 64)
+          L.win[k][row * W + col] = Z[k][(size_t)(wa0[k] + row) * ns + wr0[k] + col];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (!b[k].valid) continue;
+    if (staged[k])
+      sinc_adj(&L.win[k][(ia[k] - 3 - wa0[k]) * wc[k] + (ir[k] - 3 - wr0[k])], wc[k], fr[k], fa[k],
+               sp[k], cp[k], b[k]);
+    else
+      sinc_adj(Z[k] + (size_t)(ia[k] - 3) * G[k]->samples + (ir[k] - 3), G[k]->samples, fr[k],
+               fa[k], sp[k], cp[k], b[k]);
+  }
+  double gp[6] = {0, 0, 0, 0, 0, 0};
+  if (active) {
+    const long idx = (long)(r / rl) * nc_ml + c / cl;
+    const float inv_n = 1.0f / (float)(rl * cl);
+    float2 gI = g_ifg ? g_ifg[idx] : make_float2(0.f, 0.f);
+    gI.x *= inv_n;
+    gI.y *= inv_n;
+    const float g1 = g_pr ? 2.0f * g_pr[idx] * inv_n : 0.f;
+    const float g2 = g_ps ? 2.0f * g_ps[idx] * inv_n : 0.f;
+    const float2 z0 = b[0].z, z1 = b[1].z;
+    // G_r = conj(gI) z_s + g1 z_r ; G_s = gI z_r + g2 z_s
+    float2 Gref = make_float2(gI.x * z1.x + gI.y * z1.y + g1 * z0.x,
+                              gI.x * z1.y - gI.y * z1.x + g1 * z0.y);
+    float2 Gsec = make_float2(gI.x * z0.x - gI.y * z0.y + g2 * z1.x,
+                              gI.x * z0.y + gI.y * z0.x + g2 * z1.y);
+    double gh = burst_chain(*gpr, none, b[0], Gref, nullptr) + burst_chain(*gps, ops, b[1], Gsec, gp);
+    if (g_dh) g_dh[(size_t)r * gr.ncol + c] = (float)gh;
+  }
+  // block sum of the 6 partials: wavefront butterflies, then the
+  // wavefronts in order (fixed order)
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double v = gp[k];
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    if ((tid & 63) == 0) L.red[tid >> 6][k] = v;
+  }
+  __syncthreads();
+  if (tid < 6) {
+    double v = 0.0;
+    for (int w = 0; w < kBwdTile * kBwdTile / 64; ++w) v += L.red[w][tid];
+    partial[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 6 + tid] = v;
+  }
+}
+
+// Sum of n records of 6 partials in two fixed-order levels: block b of
+// the first level sums records [b chunk, (b + 1) chunk) (thread t adds
+// records t, t + 256, ... of the chunk, then a fixed tree over the
+// threads); one block then sums the block results the same way.
+constexpr int kReduceThreads = 256;
+constexpr int kReduceBlocks = 256;
+
+__global__ __launch_bounds__(kReduceThreads) void k_reduce6(const double *__restrict__ in, long n,
+                                                            long chunk, double *out) {
+  __shared__ double red[6][kReduceThreads];
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  const long b0 = (long)blockIdx.x * chunk;
+  const long b1 = b0 + chunk < n ? b0 + chunk : n;
+  for (long b = b0 + threadIdx.x; b < b1; b += kReduceThreads)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] += in[(size_t)b * 6 + k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int w = kReduceThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // Coherence adjoint, step 1: per output pixel q the coefficients of the
@@ -359,6 +546,54 @@ __global__ void k_coh_bwd_box(const float2 *__restrict__ cA, const float *__rest
   if (g2) g2[o] = s2;
 }
 
+// Tiled forms of the two steps (win / 2 <= kCohMaxHalo): the window sums
+// come from an LDS tile with halo (the forward's staging); same sums, same
+// order.
+__global__ __launch_bounds__(kCohTX * kCohTY) void k_coh_bwd_coef_tiled(
+    const float2 *__restrict__ ifg, const float *__restrict__ p1, const float *__restrict__ p2,
+    int nr, int nc, int win, const float *__restrict__ g, float2 *__restrict__ cA,
+    float *__restrict__ cB, float *__restrict__ cC) {
+  __shared__ CohTile L;
+  int r, c;
+  float sx, sy, s1, s2;
+  if (!window_sums_tile(ifg, p1, p2, nr, nc, win, blockIdx.x, blockIdx.y, L, r, c, sx, sy, s1,
+                        s2))
+    return;
+  const size_t o = (size_t)r * nc + c;
+  const float gq = g[o];
+  float2 ra = make_float2(0.f, 0.f);
+  float rb = 0.f, rc = 0.f;
+  float den = sqrtf(s1 * s2);
+  float mag = sqrtf(sx * sx + sy * sy);
+  if (den > 0.f && mag > 0.f) {
+    float gam = mag / den;
+    if (gam <= 1.0f) {
+      float s = gq / (mag * den);
+      ra = make_float2(s * sx, s * sy);
+      rb = -0.5f * gq * gam / s1;
+      rc = -0.5f * gq * gam / s2;
+    }
+  }
+  cA[o] = ra;
+  cB[o] = rb;
+  cC[o] = rc;
+}
+
+__global__ __launch_bounds__(kCohTX * kCohTY) void k_coh_bwd_box_tiled(
+    const float2 *__restrict__ cA, const float *__restrict__ cB, const float *__restrict__ cC,
+    int nr, int nc, int win, float2 *__restrict__ gI, float *__restrict__ g1,
+    float *__restrict__ g2) {
+  __shared__ CohTile L;
+  int r, c;
+  float sx, sy, s1, s2;
+  if (!window_sums_tile(cA, cB, cC, nr, nc, win, blockIdx.x, blockIdx.y, L, r, c, sx, sy, s1, s2))
+    return;
+  const size_t o = (size_t)r * nc + c;
+  if (gI) gI[o] = make_float2(sx, sy);
+  if (g1) g1[o] = s1;
+  if (g2) g2[o] = s2;
+}
+
 }  // namespace
 
 namespace s1ifg {
@@ -390,11 +625,16 @@ int s1ifg_pair_backward(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geo
   int nr = gr.nrow / rl, nc = gr.ncol / cl;
   if (nr < 1 || nc < 1) return set_error(S1IFG_EINVAL, "grid smaller than one look cell");
   long nml = (long)nr * nc;
-  unsigned blocks = (unsigned)((nml + kAdjBlock - 1) / kAdjBlock);
+  const bool tiles = interp == S1IFG_INTERP_SINC;
+  const dim3 tgrid((unsigned)((nc * cl + kBwdTile - 1) / kBwdTile),
+                   (unsigned)((nr * rl + kBwdTile - 1) / kBwdTile));
+  unsigned blocks = tiles ? tgrid.x * tgrid.y : (unsigned)((nml + kAdjBlock - 1) / kAdjBlock);
   RowTab *rows;
   ColTab *cols;
   void *part = nullptr;
-  if ((rc = grid_tables(ctx, gr, dd, &rows, &cols, sizeof(double) * 6 * (size_t)blocks, &part)))
+  // per-block partials, then kReduceBlocks first-level sums
+  if ((rc = grid_tables(ctx, gr, dd, &rows, &cols,
+                        sizeof(double) * 6 * ((size_t)blocks + kReduceBlocks), &part)))
     return rc;
   if (g_dh) {
     long n = (long)gr.nrow * gr.ncol;
@@ -408,14 +648,18 @@ int s1ifg_pair_backward(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geo
                        (const float2 *)ref_deramped, (const float2 *)sec_deramped, rl, cl, nr, nc,
                        (const float2 *)g_ifg, g_pow_ref, g_pow_sec, (double *)part, g_dh);
   else
-    hipLaunchKernelGGL(k_pair_bwd<S1IFG_INTERP_SINC>, dim3(blocks), dim3(kAdjBlock), 0,
-                       ctx->stream, g_ref->d_dev, g_sec->d_dev, op, gr, dd, rows, cols, dh,
+    hipLaunchKernelGGL(k_pair_bwd_tiles, tgrid, dim3(kBwdTile * kBwdTile), 0, ctx->stream,
+                       g_ref->d_dev, g_sec->d_dev, op, gr, dd, rows, cols, dh,
                        (const float2 *)ref_deramped, (const float2 *)sec_deramped, rl, cl, nr, nc,
                        (const float2 *)g_ifg, g_pow_ref, g_pow_sec, (double *)part, g_dh);
   S1_LAUNCH_CHECK("k_pair_bwd");
   if (g_offpoly) {
-    hipLaunchKernelGGL(k_reduce6, dim3(1), dim3(64), 0, ctx->stream, (const double *)part,
-                       (int)blocks, g_offpoly);
+    double *lvl = (double *)part + 6 * (size_t)blocks;
+    const long chunk = ((long)blocks + kReduceBlocks - 1) / kReduceBlocks;
+    hipLaunchKernelGGL(k_reduce6, dim3(kReduceBlocks), dim3(kReduceThreads), 0, ctx->stream,
+                       (const double *)part, (long)blocks, chunk, lvl);
+    hipLaunchKernelGGL(k_reduce6, dim3(1), dim3(kReduceThreads), 0, ctx->stream,
+                       (const double *)lvl, (long)kReduceBlocks, (long)kReduceBlocks, g_offpoly);
     S1_LAUNCH_CHECK("k_reduce6");
   }
   return S1IFG_OK;
@@ -445,6 +689,15 @@ int s1ifg_coherence_backward(s1ifg_ctx *ctx, const s1ifg_cf32 *ifg_ml, const flo
   float2 *cA = (float2 *)ws;
   float *cB = (float *)((char *)ws + ((n * 8 + 255) & ~(size_t)255));
   float *cC = (float *)((char *)cB + ((n * 4 + 255) & ~(size_t)255));
+  if (win / 2 <= kCohMaxHalo) {
+    hipLaunchKernelGGL(k_coh_bwd_coef_tiled, grd, blk, 0, ctx->stream, (const float2 *)ifg_ml,
+                       pow_ref, pow_sec, nrow, ncol, win, g_coh, cA, cB, cC);
+    S1_LAUNCH_CHECK("k_coh_bwd_coef_tiled");
+    hipLaunchKernelGGL(k_coh_bwd_box_tiled, grd, blk, 0, ctx->stream, cA, cB, cC, nrow, ncol,
+                       win, (float2 *)g_ifg, g_pow_ref, g_pow_sec);
+    S1_LAUNCH_CHECK("k_coh_bwd_box_tiled");
+    return S1IFG_OK;
+  }
   hipLaunchKernelGGL(k_coh_bwd_coef, grd, blk, 0, ctx->stream, (const float2 *)ifg_ml, pow_ref,
                      pow_sec, nrow, ncol, win, g_coh, cA, cB, cC);
   S1_LAUNCH_CHECK("k_coh_bwd_coef");
