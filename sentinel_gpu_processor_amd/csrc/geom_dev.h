@@ -9,23 +9,25 @@ namespace s1ifg {
 // Per-stencil record in the orbit buffer: [t_k, coef[24], pad...].
 constexpr int kOrbRec = 32;
 
-// Position, velocity and acceleration from the polynomial form of the
-// 4-node Hermite interpolation (orbit.cu:170-245).  Stencil index: truncated
-// (t - t0)/(t1 - t0) clamped to [1, n-3] (orbit.cu:229-232).
-__device__ __forceinline__ void orbit_eval(const OrbitDev &o, double t, double x[3],
-                                           double v[3], double a[3]) {
-  int k = (int)((t - o.t0) * o.inv_h);
-  k = k > 1 ? k : 1;
-  k = k < o.n - 3 ? k : o.n - 3;
-  const double *rec = o.coef + (size_t)(k - 1) * kOrbRec;
-  const double inv_h = o.inv_h;
+// Horner evaluation of one stencil record rec = [t_k, coef[24]] (P is a
+// pointer to const double in any address space).  p'' and p' start at 0
+// and p' picks up c[7] first: their first steps fma(0, s, x) = x are
+// taken directly (same values).
+template <typename P>
+__device__ __forceinline__ void orbit_horner(P rec, double t, double inv_h, double x[3],
+                                             double v[3], double a[3]) {
   const double s = (t - rec[0]) * inv_h;
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
-    const double *c = rec + 1 + comp * 8;
-    double p = c[7], dp = 0.0, ddp = 0.0;
+    P c = rec + 1 + comp * 8;
+    // m = 6: ddp = 0, dp = c7, p = c7 s + c6;  m = 5: ddp = 2 c7
+    double dp = c[7];
+    double p = fma(c[7], s, c[6]);
+    double ddp = 2.0 * dp;
+    dp = fma(dp, s, p);
+    p = fma(p, s, c[5]);
 #pragma unroll
-    for (int m = 6; m >= 0; --m) {
+    for (int m = 4; m >= 0; --m) {
       ddp = fma(ddp, s, 2.0 * dp);
       dp = fma(dp, s, p);
       p = fma(p, s, c[m]);
@@ -33,6 +35,27 @@ __device__ __forceinline__ void orbit_eval(const OrbitDev &o, double t, double x
     x[comp] = p;
     v[comp] = dp * inv_h;
     a[comp] = ddp * inv_h * inv_h;
+  }
+}
+
+// Position, velocity and acceleration from the polynomial form of the
+// 4-node Hermite interpolation (orbit.cu:170-245).  Stencil index: truncated
+// (t - t0)/(t1 - t0) clamped to [1, n-3] (orbit.cu:229-232).  When the
+// wavefront's active lanes share the stencil (nearly always: a wavefront
+// spans milliseconds of azimuth time, the stencils seconds) the record is
+// read with scalar loads through the constant address space; otherwise per
+// lane.  Both read the same record, so the values are the same.
+__device__ __forceinline__ void orbit_eval(const OrbitDev &o, double t, double x[3],
+                                           double v[3], double a[3]) {
+  int k = (int)((t - o.t0) * o.inv_h);
+  k = k > 1 ? k : 1;
+  k = k < o.n - 3 ? k : o.n - 3;
+  const int k0 = __builtin_amdgcn_readfirstlane(k);
+  if (__all(k == k0)) {
+    typedef const __attribute__((address_space(4))) double *CP;
+    orbit_horner((CP)o.coef + (size_t)(k0 - 1) * kOrbRec, t, o.inv_h, x, v, a);
+  } else {
+    orbit_horner(o.coef + (size_t)(k - 1) * kOrbRec, t, o.inv_h, x, v, a);
   }
 }
 
