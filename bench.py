@@ -73,6 +73,31 @@ def stage_bytes(L, N, dem_px, rl, cl, dem_bytes=4):
     }
 
 
+PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                           "c2_burst_pair_hbm_traffic.md")
+GOLD_MFMA_FLOPS_PER_PATCH = 112 * 32 * 32 * 16 * 2  # v_mfma_f32_32x32x16_f16 per 32x32 patch
+FP16_DENSE_PEAK = 2.5e15
+
+
+def profiled(kind):
+    """{stage: value} from the `kind` line of the committed rocprofv3 PMC
+    summary of the 1500 x 25000 pair (HBM traffic per launch, VALU issue
+    fraction); {} when absent."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith(kind + " "):
+                    return {k: float(v) for k, v in (t.split("=") for t in line.split()[1:])}
+    except OSError:
+        pass
+    return {}
+
+
+def gold_patches(nrow, ncol, n=32, step=8):
+    """Patch count of the Goldstein grid (origins min(j step, size - n))."""
+    return (-(-(nrow - n) // step) + 1) * (-(-(ncol - n) // step) + 1)
+
+
 def b_alg(L, N, dem_px, rl, cl, dem_bytes=4):
     n = L * N
     nml = (L // rl) * (N // cl)
@@ -530,10 +555,19 @@ def calibrate_copy(nbytes=1 << 30, reps=3):
     import torch
     src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
     dst = torch.empty_like(src)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    dst.copy_(src)
+    e0.record()
     for _ in range(reps):
         dst.copy_(src)
-    torch.cuda.synchronize()
-    print(f"[bench] calibration copy: {reps} x {nbytes} bytes", file=sys.stderr, flush=True)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    print(f"[bench] calibration copy: {reps + 1} x {nbytes} bytes, {ms:.3f} ms each "
+          f"({gbs:.0f} GB/s read + write)", file=sys.stderr, flush=True)
+    return gbs
 
 
 def cpu_baseline(sample):
@@ -598,14 +632,29 @@ def main():
     # (the batched form is one graph launch of the same device code)
     r = res["fused"] if "fused" in res else res[best]
     dom = max(r["stages"], key=r["stages"].get)
-    per_launch = {"deramp": 2, "geocode": 2}.get(dom, 1)
-    t_launch = r["stages"][dom] / per_launch * 1e-3
-    achieved = sb[dom] / t_launch / 1e9
-    roof = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK / 1e9,
-                unit="GB/s", frac=round(achieved * 1e9 / HBM_PEAK, 4), traffic=None,
-                alg_bytes_per_launch=sb[dom])
+    traffic = profiled("traffic_bytes")
+    valu = profiled("valu_issue_busy")
+    kernels = {}
+    for st, ms_st in r["stages"].items():
+        per_launch = {"deramp": 2, "geocode": 2}.get(st, 1)
+        t_launch = ms_st / per_launch * 1e-3
+        ach = sb[st] / t_launch / 1e9
+        kernels[st] = dict(ms_per_launch=round(t_launch * 1e3, 4), alg_bytes_per_launch=sb[st],
+                           achieved_gbs=round(ach, 1), hbm_frac=round(ach * 1e9 / HBM_PEAK, 4),
+                           traffic=traffic.get(st), valu_issue_busy=valu.get(st))
+        if st == "goldstein" and res["_gold_mode"] == "fp16_split":
+            fl = gold_patches(L // rl, Nr // cl) * GOLD_MFMA_FLOPS_PER_PATCH
+            kernels[st].update(mfma_tflops=round(fl / t_launch / 1e12, 1),
+                               mfma_frac=round(fl / t_launch / FP16_DENSE_PEAK, 4))
+    k = kernels[dom]
+    roof = dict(kernel=dom, bound="hbm", achieved=k["achieved_gbs"], peak=HBM_PEAK / 1e9,
+                unit="GB/s", frac=k["hbm_frac"], traffic=k["traffic"],
+                alg_bytes_per_launch=sb[dom], valu_issue_busy=k["valu_issue_busy"],
+                limiter=("VALU issue: per-pixel FP64 geometry and FP32 8x8 sinc taps"
+                         if dom in ("pair", "geocode") else None),
+                kernels=kernels)
     if args.calibrate_copy:
-        calibrate_copy()
+        roof["copy_gbs"] = round(calibrate_copy(), 1)
     iw = None
     if not args.no_iw:
         try:
