@@ -156,29 +156,39 @@ __device__ __forceinline__ float reduce_phase(double ph) {
 
 // 8-tap Hann-windowed sinc weights for fractional offset f in [0,1):
 // w_k ~ sinc(f - k) * cos^2(pi (f - k)/8), k = -3..4, normalised to sum 1.
+// With sin(pi (f - k)) = (-1)^k sin(pi f), sinc(f - k) is
+// (-1)^k sin(pi f) / (pi P) * prod_{j != k} (f - j), P = prod_j (f - j); the
+// factor sin(pi f) / (pi P) is common to all taps and cancels in the
+// normalisation, so w_k = q_k / sum_j q_j with
+// q_k = (-1)^k cos^2(pi (f - k)/8) prod_{j != k} (f - j) (prefix and suffix
+// products; no per-tap reciprocal, and f = 0 gives w = delta_k0 directly).
 __device__ __forceinline__ void sinc_weights(float f, float w[8]) {
-  // hardware sin/cos take revolutions: sin(pi f) = sin(2 pi (f/2))
-  const float s = __builtin_amdgcn_sinf(0.5f * f);
+  // hardware sin/cos take revolutions: sin(pi f / 4) = sin(2 pi (f/8))
   const float s4 = __builtin_amdgcn_sinf(0.125f * f);
   const float c4 = __builtin_amdgcn_cosf(0.125f * f);
   // cos(pi k/4), sin(pi k/4) for k = -3..4
   const float ck[8] = {-0.70710678f, 0.0f, 0.70710678f, 1.0f, 0.70710678f, 0.0f, -0.70710678f, -1.0f};
   const float sk[8] = {-0.70710678f, -1.0f, -0.70710678f, 0.0f, 0.70710678f, 1.0f, 0.70710678f, 0.0f};
+  float d[8], pre[8], suf[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) d[i] = f - (float)(i - 3);
+  pre[0] = 1.0f;
+#pragma unroll
+  for (int i = 1; i < 8; ++i) pre[i] = pre[i - 1] * d[i - 1];
+  suf[7] = 1.0f;
+#pragma unroll
+  for (int i = 6; i >= 0; --i) suf[i] = suf[i + 1] * d[i + 1];
   float sum = 0.f;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    int k = i - 3;
-    float x = f - (float)k;
-    // sin(pi (f - k)) = (-1)^k sin(pi f)
-    float sn = (k & 1) ? -s : s;
-    float sc = (x == 0.0f) ? 1.0f : sn * __builtin_amdgcn_rcpf(3.14159265358979f * x);
     // cos(pi (f-k)/4) = cos(pi f/4) cos(pi k/4) + sin(pi f/4) sin(pi k/4)
-    float cw = c4 * ck[i] + s4 * sk[i];
-    float wv = sc * 0.5f * (1.0f + cw);
-    w[i] = wv;
-    sum += wv;
+    const float cw = c4 * ck[i] + s4 * sk[i];
+    const float win = 0.5f * (1.0f + cw);
+    const float q = pre[i] * suf[i] * win;
+    w[i] = (i & 1) ? q : -q;  // (-1)^k, k = i - 3
+    sum += w[i];
   }
-  float inv = __builtin_amdgcn_rcpf(sum);
+  const float inv = __builtin_amdgcn_rcpf(sum);
 #pragma unroll
   for (int i = 0; i < 8; ++i) w[i] *= inv;
 }
