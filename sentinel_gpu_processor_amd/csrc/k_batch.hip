@@ -179,7 +179,7 @@ __global__ void kb_reset(unsigned long long *ctr, int n) {
 // bursts (cache reuse) whatever the grid size; the next tile is fetched
 // while the current one runs.  A grid with a block per tile runs one tile
 // per block in block order without the queue.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kb_pair_tiles(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_pair_tiles(
     const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, int rl, int cl,
     unsigned long long *ctr) {
   __shared__ PairLds L;

@@ -110,7 +110,7 @@ __global__ void k_pair(const GeomDev *__restrict__ gpr, const GeomDev *__restric
 
 // Sinc pair kernel with LDS-staged burst windows: one 16 x 16 tile of look
 // cells per block (rl * cl <= kPairMaxPix).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pair_tiles(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pair_tiles(
     const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr,
     DemDev dem, const RowTab *rows, const ColTab *cols, const float *dh,
     const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl, int cl, int nr_ml,

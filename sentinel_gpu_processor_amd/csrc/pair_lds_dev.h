@@ -22,7 +22,7 @@
 namespace s1ifg {
 
 constexpr int kPairTile = 16;        // look cells per tile side
-constexpr int kPairWin = 3328;       // staged samples per burst (26 KB)
+constexpr int kPairWin = 2432;       // staged samples per burst (19 KB; 4 blocks per CU)
 constexpr int kPairMaxPix = 4;       // target pixels per look cell on this path
 
 struct PairLds {
