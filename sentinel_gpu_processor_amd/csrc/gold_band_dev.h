@@ -141,4 +141,107 @@ __device__ __forceinline__ float2 band_sum_pixel(const float2 *__restrict__ part
   return make_float2(ax / ws, ay / ws);
 }
 
+// band_sum_pixel over a tile of kSumTX x kSumTY output pixels (block
+// kSumTX x kSumTY): the factors that depend only on the column (the column
+// weight terms and the <= 2 chunks covering it) or only on the row (the row
+// weight terms and the <= 5 bands covering it: 4, plus the last band whose
+// origin is clamped to nrow - 32) are computed once per tile
+// column / row into LDS; every pixel then adds its partials in the same
+// order as band_sum_pixel, so the result is the same.  Contains block
+// barriers (the last one lets the caller reuse L).
+constexpr int kSumTX = 64, kSumTY = 4;
+
+struct BandSumLds {
+  float cx[kSumTX], sx[kSumTX];
+  int nch[kSumTX], ch[kSumTX][2], xo[kSumTX][2];
+  float cy[kSumTY], sy[kSumTY];
+  int nb[kSumTY], pi[kSumTY][5], ty[kSumTY][5];
+};
+
+__device__ __forceinline__ void band_sum_tile(const float2 *__restrict__ partial, int nrow,
+                                              int ncol, int pi_n, int pj_n, int tile_x,
+                                              int tile_y, float2 *__restrict__ out,
+                                              BandSumLds &L) {
+  const int step = kGoldN / 4;
+  const int nchunk = band_nchunk(pj_n);
+  const int tx = threadIdx.x, tyi = threadIdx.y;
+  const int x = tile_x * kSumTX + tx, y = tile_y * kSumTY + tyi;
+  if (tyi == 0) {
+    // column terms of x
+    float cx = 0.f, sx = 0.f;
+    int n = 0;
+    if (x < ncol) {
+      const int pj_lo = x - (kGoldN - 1) > 0 ? (x - (kGoldN - 1) + step - 1) / step : 0;
+      for (int pj = pj_lo; pj < pj_n; ++pj) {
+        const int o = gold_origin(pj, ncol);
+        if (o > x) break;
+        if (x >= o + kGoldN) continue;
+        const int t = x - o;
+        cx += 1.f;
+        sx += (float)(t < kGoldN / 2 ? t : kGoldN - 1 - t);
+      }
+      const int jx = x >= gold_origin(pj_n - 1, ncol) ? pj_n - 1 : min(x / step, pj_n - 1);
+      const int c_hi = jx / kBandCW;
+      const int c_lo = c_hi > 0 ? c_hi - 1 : 0;
+      for (int c = c_lo; c <= c_hi && c < nchunk; ++c) {
+        const int a = gold_origin(c * kBandCW, ncol);
+        const int b = gold_origin(min(c * kBandCW + kBandCW, pj_n) - 1, ncol) + kGoldN;
+        if (x < a || x >= b) continue;
+        L.ch[tx][n] = c;
+        L.xo[tx][n] = x - a;
+        ++n;
+      }
+    }
+    L.cx[tx] = cx;
+    L.sx[tx] = sx;
+    L.nch[tx] = n;
+  } else if (tyi == 1 && tx < kSumTY) {
+    // row terms of row tile_y * kSumTY + tx
+    const int yy = tile_y * kSumTY + tx;
+    float cy = 0.f, sy = 0.f;
+    int n = 0;
+    if (yy < nrow) {
+      const int pi_lo = yy - (kGoldN - 1) > 0 ? (yy - (kGoldN - 1) + step - 1) / step : 0;
+      for (int p = pi_lo; p < pi_n; ++p) {
+        const int o = gold_origin(p, nrow);
+        if (o > yy) break;
+        if (yy >= o + kGoldN) continue;
+        const int t = yy - o;
+        cy += 1.f;
+        sy += (float)(t < kGoldN / 2 ? t : kGoldN - 1 - t);
+        if (n < 5) {  // always: origins step by 8, only the last one is clamped
+          L.pi[tx][n] = p;
+          L.ty[tx][n] = t;
+          ++n;
+        }
+      }
+    }
+    L.cy[tx] = cy;
+    L.sy[tx] = sy;
+    L.nb[tx] = n;
+  }
+  __syncthreads();
+  if (x < ncol && y < nrow) {
+    const float ws = L.cx[tx] * L.sy[tyi] + L.cy[tyi] * L.sx[tx];
+    float2 r = make_float2(0.f, 0.f);
+    if (ws > 0.f) {
+      float ax = 0.f, ay = 0.f;
+      const int nb = L.nb[tyi], nch = L.nch[tx];
+      for (int i = 0; i < nb; ++i) {
+        const size_t rowb = (size_t)L.pi[tyi][i] * nchunk;
+        const int t = L.ty[tyi][i];
+        for (int k = 0; k < nch; ++k) {
+          const float2 v =
+              partial[((rowb + L.ch[tx][k]) * kGoldN + t) * kBandCols + L.xo[tx][k]];
+          ax += v.x;
+          ay += v.y;
+        }
+      }
+      r = make_float2(ax / ws, ay / ws);
+    }
+    out[(size_t)y * ncol + x] = r;
+  }
+  __syncthreads();
+}
+
 }  // namespace s1ifg

@@ -247,19 +247,23 @@ __global__ __launch_bounds__(128) void kb_gold_fp32(const S1C PairDev *__restric
 __global__ __launch_bounds__(256) void kb_gather(const S1C PairDev *__restrict__ P,
                                                  const S1C long *__restrict__ off, int n,
                                                  int band) {
+  __shared__ BandSumLds L;
   const long total = off[n];
   int ph = -1;
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
     const int p = ph = find_item(off, n, t, ph);
     const PairDev &d = *(const PairDev *)&P[p];
     const long lt = t - off[p];
-    const int tiles_x = (d.nc_ml + 63) / 64;
+    const int tiles_x = (d.nc_ml + kSumTX - 1) / kSumTX;
     const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
-    const int x = tx * 64 + threadIdx.x, y = ty * 4 + threadIdx.y;
+    if (band) {
+      band_sum_tile(d.patches, d.nr_ml, d.nc_ml, d.pi_n, d.pj_n, tx, ty, d.filt, L);
+      continue;
+    }
+    const int x = tx * kSumTX + threadIdx.x, y = ty * kSumTY + threadIdx.y;
     if (x < d.nc_ml && y < d.nr_ml)
       d.filt[(size_t)y * d.nc_ml + x] =
-          band ? band_sum_pixel(d.patches, d.nr_ml, d.nc_ml, d.pi_n, d.pj_n, y, x)
-               : gold_gather_pixel(d.patches, d.nr_ml, d.nc_ml, d.pi_n, d.pj_n, y, x);
+          gold_gather_pixel(d.patches, d.nr_ml, d.nc_ml, d.pi_n, d.pj_n, y, x);
   }
 }
 

@@ -23,12 +23,11 @@ __global__ __launch_bounds__(64 * kBandWaves) __attribute__((amdgpu_waves_per_eu
               partial, L.buf[wave]);
 }
 
-__global__ void k_gold_band_sum(const float2 *__restrict__ partial, int nrow, int ncol, int pi_n,
-                                int pj_n, float2 *__restrict__ out) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  if (x >= ncol || y >= nrow) return;
-  out[(size_t)y * ncol + x] = band_sum_pixel(partial, nrow, ncol, pi_n, pj_n, y, x);
+__global__ __launch_bounds__(kSumTX * kSumTY) void k_gold_band_sum(
+    const float2 *__restrict__ partial, int nrow, int ncol, int pi_n, int pj_n,
+    float2 *__restrict__ out) {
+  __shared__ BandSumLds L;
+  band_sum_tile(partial, nrow, ncol, pi_n, pj_n, blockIdx.x, blockIdx.y, out, L);
 }
 
 }  // namespace
@@ -59,7 +58,7 @@ int goldstein_band(s1ifg_ctx *ctx, const float2 *in, int nrow, int ncol, float a
   hipLaunchKernelGGL(k_gold_band, dim3(blocks), dim3(64 * kBandWaves), 0, ctx->stream, in, nrow, ncol, pi_n,
                      pj_n, alpha, partial);
   S1_LAUNCH_CHECK("k_gold_band");
-  dim3 blk(64, 4), grd((ncol + 63) / 64, (nrow + 3) / 4);
+  dim3 blk(kSumTX, kSumTY), grd((ncol + kSumTX - 1) / kSumTX, (nrow + kSumTY - 1) / kSumTY);
   hipLaunchKernelGGL(k_gold_band_sum, grd, blk, 0, ctx->stream, partial, nrow, ncol, pi_n, pj_n,
                      out);
   S1_LAUNCH_CHECK("k_gold_band_sum");
