@@ -62,6 +62,13 @@ __device__ __forceinline__ int kappa(int c, int h, int j) {
   return 16 * c + 8 * (j >> 2) + 4 * h + (j & 3);
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// -v for 8 packed fp16 (sign bits flipped, one XOR per dword).
+__device__ __forceinline__ half8 neg8(half8 v) {
+  return __builtin_bit_cast(half8, __builtin_bit_cast(u32x4, v) ^ 0x80008000u);
+}
+
 // hi + lo split of v (|v| < 2^16).  lo is one mixed-precision FMA
 // v * 1 - hi per value (the 1 is hidden from constant folding so that the
 // FMA is not rewritten as a subtraction with separate conversions).
@@ -140,7 +147,7 @@ __device__ __forceinline__ void cgemm(const half8 arh[2], const half8 arl[2], co
   for (int c = 0; c < 2; ++c) {
     if (!CONJ) {
       // Cr = Ar Fr - Ai Fi, Ci = Ar Fi + Ai Fr
-      const half8 nih = -aih[c], nil = -ail[c];
+      const half8 nih = neg8(aih[c]), nil = neg8(ail[c]);
       re = mfma(arl[c], b.rh[c], re);
       im = mfma(arl[c], b.ih[c], im);
       re = mfma(arh[c], b.rl[c], re);
@@ -155,7 +162,7 @@ __device__ __forceinline__ void cgemm(const half8 arh[2], const half8 arl[2], co
       im = mfma(aih[c], b.rh[c], im);
     } else {
       // Cr = Ar Fr + Ai Fi, Ci = Ai Fr - Ar Fi
-      const half8 nrh = -arh[c], nrl = -arl[c];
+      const half8 nrh = neg8(arh[c]), nrl = neg8(arl[c]);
       re = mfma(arl[c], b.rh[c], re);
       im = mfma(ail[c], b.rh[c], im);
       re = mfma(arh[c], b.rl[c], re);
