@@ -245,7 +245,7 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 
   // smoothing of |Z|
   float m[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) m[r] = sqrtf(zr[r] * zr[r] + zi[r] * zi[r]);
+  for (int r = 0; r < 16; ++r) m[r] = __builtin_amdgcn_sqrtf(zr[r] * zr[r] + zi[r] * zi[r]);
   half8 mh[2], ml[2];
   split16(m, mh, ml);
   f32x16 t1 = {}, t2 = {};
@@ -291,7 +291,9 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 
   for (int r = 0; r < 16; ++r) {
     float sn = s[r] * inv;
     if (sn > 0.f) {  // powf(sn, alpha) > 0 only here; Z is kept elsewhere
-      float w = __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(sn));
+      // alpha = 0.5 (the usual Goldstein exponent) is one square root
+      float w = alpha == 0.5f ? __builtin_amdgcn_sqrtf(sn)
+                              : __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(sn));
       zr[r] *= w;
       zi[r] *= w;
     }
