@@ -154,6 +154,17 @@ __device__ __forceinline__ float reduce_phase(double ph) {
   return (float)fma(-k, kTwoPi, ph);
 }
 
+// sin and cos of a phase: reduced in FP64 to [-1/2, 1/2] revolutions
+// (|ph| ~ 2e8 rad keeps 2e-8 rad of resolution), then the hardware sine and
+// cosine, which take revolutions (absolute error ~1e-6, far below the
+// resampling error; the library sincosf would redo the range reduction).
+__device__ __forceinline__ void sincos_phase(double ph, float &s, float &c) {
+  const double rv = ph * (1.0 / kTwoPi);
+  const float r = (float)(rv - rint(rv));
+  s = __builtin_amdgcn_sinf(r);
+  c = __builtin_amdgcn_cosf(r);
+}
+
 // 8-tap Hann-windowed sinc weights for fractional offset f in [0,1):
 // w_k ~ sinc(f - k) * cos^2(pi (f - k)/8), k = -3..4, normalised to sum 1.
 // With sin(pi (f - k)) = (-1)^k sin(pi f), sinc(f - k) is
@@ -233,8 +244,7 @@ __device__ __forceinline__ bool geocode_geom(const GeomDev &g, const OffPoly &op
   double eta = -g.lines * g.dtaz / 2. + (t - g.tstart) + daz * g.dtaz;
   double etadiff = eta - etaref;
   double reramp = -kPi * kt * etadiff * etadiff;
-  float ph = reduce_phase(4.0 * kPi / g.wvl * rng - reramp);
-  sincosf(ph, &sp, &cp);
+  sincos_phase(4.0 * kPi / g.wvl * rng - reramp, sp, cp);
   return true;
 }
 

@@ -132,8 +132,7 @@ __device__ __forceinline__ bool burst_geom_adj(const GeomDev &g, const OffPoly &
   double ed = eta - etaref;
   o.R_rp = -kPi * (dkt * ed * ed - 2.0 * kt * ed * detaref);
   o.R_eta = -2.0 * kPi * kt * ed;
-  float ph = reduce_phase(4.0 * kPi / g.wvl * rng + kPi * kt * ed * ed);
-  sincosf(ph, &sp, &cp);
+  sincos_phase(4.0 * kPi / g.wvl * rng + kPi * kt * ed * ed, sp, cp);
   // height partials
   double vn = v[0] * nrm[0] + v[1] * nrm[1] + v[2] * nrm[2];
   double den = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] - (dr[0] * a[0] + dr[1] * a[1] + dr[2] * a[2]);
