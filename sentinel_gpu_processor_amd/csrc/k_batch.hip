@@ -86,44 +86,26 @@ __global__ void kb_tables(const S1C PairDev *__restrict__ P, const S1C long *__r
   }
 }
 
-// Deramp of every distinct burst of the batch, the element code of
-// k_deramp.
-// Deramp work item: one burst line x kDerampChunk sample pairs (one pair
-// per thread).
+// Deramp of every distinct burst of the batch (deramp_tile).  Work item:
+// kDerampRows burst lines x kDerampChunk sample pairs (one pair per thread).
 constexpr int kDerampChunk = 256;
 
-__global__ __launch_bounds__(256) void kb_deramp(const S1C ImgDev *__restrict__ I,
-                                                 const S1C long *__restrict__ off, int nimg) {
+__global__ __launch_bounds__(kDerampChunk) void kb_deramp(const S1C ImgDev *__restrict__ I,
+                                                          const S1C long *__restrict__ off,
+                                                          int nimg) {
   const long total = off[nimg];
   int k = -1;
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
     k = find_item(off, nimg, t, k);
     const ImgDev &im = *(const ImgDev *)&I[k];
     const GeomDev &g = geo(im.g);
-    const float2 *in = im.in;
-    float2 *out = im.out;
     const int N = g.samples;
     const int nchunk = (N + 2 * kDerampChunk - 1) / (2 * kDerampChunk);
     const long lt = t - off[k];
-    const int i = (int)(lt / nchunk);
-    const int jj = (int)(lt - (long)i * nchunk) * kDerampChunk + threadIdx.x;
-    const double eta = -g.lines * g.ati / 2. + i * g.ati;
-    const size_t rowoff = (size_t)i * N;
-    const bool vec = (N % 2 == 0) && (((uintptr_t)in | (uintptr_t)out) % 16 == 0);
-    const float4 *in2 = (const float4 *)in;
-    float4 *out2 = (float4 *)out;
-    if (jj < (N + 1) / 2) {
-      int j = 2 * jj;
-      if (vec) {
-        float4 a = in2[(rowoff >> 1) + jj];
-        float2 o0 = deramp_elem(eta, g.kt[j], g.etaref[j], make_float2(a.x, a.y));
-        float2 o1 = deramp_elem(eta, g.kt[j + 1], g.etaref[j + 1], make_float2(a.z, a.w));
-        out2[(rowoff >> 1) + jj] = make_float4(o0.x, o0.y, o1.x, o1.y);
-      } else {
-        for (int q = j; q < j + 2 && q < N; ++q)
-          out[rowoff + q] = deramp_elem(eta, g.kt[q], g.etaref[q], in[rowoff + q]);
-      }
-    }
+    const int rg = (int)(lt / nchunk);
+    const bool vec = (N % 2 == 0) && (((uintptr_t)im.in | (uintptr_t)im.out) % 16 == 0);
+    deramp_tile(g, im.in, im.out, rg * kDerampRows, (int)(lt - (long)rg * nchunk) * kDerampChunk,
+                vec);
   }
 }
 
@@ -543,8 +525,9 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
   for (int st = 0; st < kNumStages; ++st) b->off[st].assign(st == kStDeramp ? nimg + 1 : n + 1, 0);
   for (int q = 0; q < nimg; ++q)
     b->off[kStDeramp][q + 1] =
-        b->off[kStDeramp][q] + (long)img_geom[q]->dev.lines *
-                                   ((img_geom[q]->dev.samples + 2 * kDerampChunk - 1) / (2 * kDerampChunk));
+        b->off[kStDeramp][q] +
+        (long)((img_geom[q]->dev.lines + kDerampRows - 1) / kDerampRows) *
+            ((img_geom[q]->dev.samples + 2 * kDerampChunk - 1) / (2 * kDerampChunk));
   for (int p = 0; p < n; ++p) {
     const PairDev &d = b->pairs[p];
     long tab = d.grid.nrow > d.grid.ncol ? d.grid.nrow : d.grid.ncol;

@@ -28,28 +28,10 @@ __global__ void k_orbit_eval(OrbitDev o, const double *t, long n, double *pos, d
 }
 
 // deramp_burst (geo2rdr.cu:138-158): phase in FP64 (eta as geo2rdr.cu:529-532),
-// reduced to [-pi, pi] before a float sincos.  Two pixels per thread.
-// in and out may alias (in-place deramp): no __restrict__ here.
-__global__ void k_deramp(GeomDev g, const float4 *in2, float4 *out2, const float2 *in1,
-                         float2 *out1, int vec) {
-  const int i = blockIdx.y;
-  const int N = g.samples;
-  const double eta = -g.lines * g.ati / 2. + i * g.ati;
-  const size_t rowoff = (size_t)i * N;
-  for (int jj = blockIdx.x * blockDim.x + threadIdx.x; jj < (N + 1) / 2;
-       jj += gridDim.x * blockDim.x) {
-    int j = 2 * jj;
-    if (vec) {
-      float4 a = in2[(rowoff >> 1) + jj];
-      float2 o0 = deramp_elem(eta, g.kt[j], g.etaref[j], make_float2(a.x, a.y));
-      float2 o1 = deramp_elem(eta, g.kt[j + 1], g.etaref[j + 1], make_float2(a.z, a.w));
-      out2[(rowoff >> 1) + jj] = make_float4(o0.x, o0.y, o1.x, o1.y);
-    } else {
-      for (int k = j; k < j + 2 && k < N; ++k) {
-        out1[rowoff + k] = deramp_elem(eta, g.kt[k], g.etaref[k], in1[rowoff + k]);
-      }
-    }
-  }
+// reduced to [-pi, pi] before a float sincos.  Block = 256 sample pairs x
+// kDerampRows rows (deramp_tile).  in and out may alias (in-place deramp).
+__global__ __launch_bounds__(256) void k_deramp(GeomDev g, const float2 *in, float2 *out, int vec) {
+  deramp_tile(g, in, out, blockIdx.y * kDerampRows, blockIdx.x * blockDim.x, vec != 0);
 }
 
 __global__ void k_geo2rdr(const GeomDev *__restrict__ gp, OffPoly op, GridDev gr, DemDev dem, const RowTab *rows,
@@ -224,8 +206,8 @@ int s1ifg_deramp(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_cf32 *in, s1if
   int bx = (pairs + 255) / 256;
   // float4 path: even row length and 16-byte aligned rows; otherwise float2
   int vec = (gd.samples % 2 == 0) && (((uintptr_t)in | (uintptr_t)out) % 16 == 0);
-  hipLaunchKernelGGL(k_deramp, dim3(bx, gd.lines), dim3(256), 0, ctx->stream, gd,
-                     (const float4 *)in, (float4 *)out, (const float2 *)in, (float2 *)out, vec);
+  hipLaunchKernelGGL(k_deramp, dim3(bx, (gd.lines + kDerampRows - 1) / kDerampRows), dim3(256),
+                     0, ctx->stream, gd, (const float2 *)in, (float2 *)out, vec);
   S1_LAUNCH_CHECK("k_deramp");
   return S1IFG_OK;
 }
