@@ -52,10 +52,11 @@ def parse():
     p.add_argument("--stack-dates", type=int, default=20)
     p.add_argument("--gold-mode", default="auto", choices=["auto", "fp32", "fp16_split"])
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-sample", default="c1")
+    p.add_argument("--cpu-sample", default="c2crop", choices=["c2crop", "c1"])
     p.add_argument("--calibrate-copy", action="store_true",
-                   help="also run a plain 1 GiB device copy (3 launches) whose HBM "
-                        "counters scale the path's FETCH_SIZE / WRITE_SIZE readings")
+                   help="(default) time a plain 1 GiB device copy: the measured streaming "
+                        "rate, and the reading that scales FETCH_SIZE / WRITE_SIZE counters")
+    p.add_argument("--no-calibrate-copy", action="store_true")
     return p.parse_args()
 
 
@@ -77,6 +78,7 @@ PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles
                            "c2_burst_pair_hbm_traffic.md")
 GOLD_MFMA_FLOPS_PER_PATCH = 112 * 32 * 32 * 16 * 2  # v_mfma_f32_32x32x16_f16 per 32x32 patch
 FP16_DENSE_PEAK = 2.5e15
+FP64_PEAK = 78.6e12
 
 
 def profiled(kind):
@@ -571,7 +573,10 @@ def calibrate_copy(nbytes=1 << 30, reps=3):
 
 
 def cpu_baseline(sample):
-    """FP64 NumPy oracle chain on a bounded sample (single process)."""
+    """FP64 NumPy oracle chain on a bounded sample (single process): "c2crop"
+    is the C2 pair's generator, DEM and offsets cut to 1500 x 1024 with the
+    benchmark's sinc resampling (~20 s of CPU work); "c1" is the reference's
+    own CPU-runnable case (bilinear)."""
     import numpy as np
 
     from oracle import geom as G
@@ -579,9 +584,15 @@ def cpu_baseline(sample):
     from oracle import ifg as I
     from sentinel_gpu_processor_amd import synthetic as S
 
-    cfg = S.CONFIGS[sample]
+    if sample == "c2crop":
+        cfg = dict(S.CONFIGS["c2"], samples=1024)
+        interp = "sinc"
+    else:
+        cfg = S.CONFIGS[sample]
+        interp = "bilinear"
     sc = S.make_scene(lines=cfg["lines"], samples=cfg["samples"], dem_shape=cfg["dem"],
-                      hmax=cfg["hmax"], seed=0, fidelity="fast", device="cuda:0")
+                      dem_kind=cfg["dem_kind"], hmax=cfg["hmax"], seed=0, fidelity="fast",
+                      device="cuda:0", misreg=cfg["offset"])
     ref = sc["ref_slc"].cpu().numpy()
     sec = sc["sec_slc"].cpu().numpy()
     t0 = time.perf_counter()
@@ -591,8 +602,9 @@ def cpu_baseline(sample):
     ss = G.deramp_setup(sc["orbit_sec"], b)
     dr = G.deramp(ref, su).astype(np.complex64)
     ds = G.deramp(sec, ss).astype(np.complex64)
-    o_r, _ = G.reproject(sc["orbit_ref"], b, su, sc["grid"], h, dr)
-    o_s, _ = G.reproject(sc["orbit_sec"], b, ss, sc["grid"], h, ds)
+    o_r, _ = G.reproject(sc["orbit_ref"], b, su, sc["grid"], h, dr, interp=interp)
+    o_s, _ = G.reproject(sc["orbit_sec"], b, ss, sc["grid"], h, ds, interp=interp,
+                         offpoly=sc["offpoly"])
     ifg = I.cpx_looks(I.conj_mul(o_r, o_s), 1, 4)
     p1 = I.cpx_looks(np.abs(o_r) ** 2, 1, 4).real
     p2 = I.cpx_looks(np.abs(o_s) ** 2, 1, 4).real
@@ -602,7 +614,7 @@ def cpu_baseline(sample):
     npx = cfg["lines"] * cfg["samples"]
     return dict(value=npx / dt / 1e6, unit="Mpix/s", cores=1, kind="port",
                 sample=f"{sample}: one {cfg['lines']}x{cfg['samples']} burst pair through the "
-                       f"FP64 NumPy oracle chain (bilinear, 1x4 looks, Goldstein 32/0.5, 5x5 "
+                       f"FP64 NumPy oracle chain ({interp}, 1x4 looks, Goldstein 32/0.5, 5x5 "
                        f"coherence) in {dt:.1f} s")
 
 
@@ -624,6 +636,7 @@ def main():
         r = res[name]
         ms = r["ms_per_step"]
         forms[name] = dict(ms_per_step=round(ms, 4), mpix_s=round(npx / (ms * 1e-3) / 1e6, 1),
+                           ml_mpix_s=round(npx / (rl * cl) / (ms * 1e-3) / 1e6, 1),
                            hbm_frac=round(balg / (ms * 1e-3) / HBM_PEAK, 4),
                            stages_ms={k: round(v, 4) for k, v in r["stages"].items()})
     best = min(forms, key=lambda k: forms[k]["ms_per_step"])
@@ -634,6 +647,7 @@ def main():
     dom = max(r["stages"], key=r["stages"].get)
     traffic = profiled("traffic_bytes")
     valu = profiled("valu_issue_busy")
+    fp64 = profiled("fp64_flops")
     kernels = {}
     for st, ms_st in r["stages"].items():
         per_launch = {"deramp": 2, "geocode": 2}.get(st, 1)
@@ -642,6 +656,9 @@ def main():
         kernels[st] = dict(ms_per_launch=round(t_launch * 1e3, 4), alg_bytes_per_launch=sb[st],
                            achieved_gbs=round(ach, 1), hbm_frac=round(ach * 1e9 / HBM_PEAK, 4),
                            traffic=traffic.get(st), valu_issue_busy=valu.get(st))
+        if st in fp64:
+            kernels[st].update(fp64_tflops=round(fp64[st] / t_launch / 1e12, 2),
+                               fp64_frac=round(fp64[st] / t_launch / FP64_PEAK, 4))
         if st == "goldstein" and res["_gold_mode"] == "fp16_split":
             fl = gold_patches(L // rl, Nr // cl) * GOLD_MFMA_FLOPS_PER_PATCH
             kernels[st].update(mfma_tflops=round(fl / t_launch / 1e12, 1),
@@ -653,7 +670,7 @@ def main():
                 limiter=("VALU issue: per-pixel FP64 geometry and FP32 8x8 sinc taps"
                          if dom in ("pair", "geocode") else None),
                 kernels=kernels)
-    if args.calibrate_copy:
+    if not args.no_calibrate_copy:
         roof["copy_gbs"] = round(calibrate_copy(), 1)
     iw = None
     if not args.no_iw:
