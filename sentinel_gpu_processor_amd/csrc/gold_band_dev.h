@@ -57,7 +57,10 @@ __device__ __forceinline__ void band_item(const mf::Consts &K, const float2 *__r
     float2 x[16];
     mf::load_patch(img, nrow, ncol, pj_n, rowbase + j, h, n, x);
     float2 out[16];
-    mf::filter_patch_regs(K, x, h, n, alpha, out);
+    if (alpha == 0.5f)
+      mf::filter_patch_regs<true>(K, x, h, n, alpha, out);
+    else
+      mf::filter_patch_regs<false>(K, x, h, n, alpha, out);
     // add the patch (each lane owns distinct (row, column) entries)
     const int ox = gold_origin(j, ncol);
     float2 *row = buf + n * kBandLd;

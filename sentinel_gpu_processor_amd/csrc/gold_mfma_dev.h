@@ -222,6 +222,9 @@ __device__ __forceinline__ void load_patch(const float2 *__restrict__ img, int n
 // Filter one patch held in x (lane layout) into its weighted result
 // w(row, col) z'/n^2 in the same layout: lane (h, n) holds row n, columns
 // 4h + 8q + t (q, t = 0..3) in out[4q + t].
+// SQRT: alpha = 0.5 (the usual Goldstein exponent), the weight is one
+// square root.
+template <bool SQRT>
 __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 x[16], int h,
                                                   int n, float alpha, float2 out[16]) {
   float xr[16], xi[16];
@@ -291,9 +294,8 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 
   for (int r = 0; r < 16; ++r) {
     float sn = s[r] * inv;
     if (sn > 0.f) {  // powf(sn, alpha) > 0 only here; Z is kept elsewhere
-      // alpha = 0.5 (the usual Goldstein exponent) is one square root
-      float w = alpha == 0.5f ? __builtin_amdgcn_sqrtf(sn)
-                              : __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(sn));
+      float w = SQRT ? __builtin_amdgcn_sqrtf(sn)
+                     : __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(sn));
       zr[r] *= w;
       zi[r] *= w;
     }
