@@ -574,8 +574,8 @@ def calibrate_copy(nbytes=1 << 30, reps=3):
 
 def cpu_baseline(sample):
     """FP64 NumPy oracle chain on a bounded sample (single process): "c2crop"
-    is the C2 pair's generator, DEM and offsets cut to 1500 x 1024 with the
-    benchmark's sinc resampling (~20 s of CPU work); "c1" is the reference's
+    is the C2 pair's generator, DEM and offsets cut to 1500 x 3072 with the
+    benchmark's sinc resampling (~15 s of CPU work); "c1" is the reference's
     own CPU-runnable case (bilinear)."""
     import numpy as np
 
@@ -585,7 +585,7 @@ def cpu_baseline(sample):
     from sentinel_gpu_processor_amd import synthetic as S
 
     if sample == "c2crop":
-        cfg = dict(S.CONFIGS["c2"], samples=1024)
+        cfg = dict(S.CONFIGS["c2"], samples=3072)
         interp = "sinc"
     else:
         cfg = S.CONFIGS[sample]
