@@ -506,7 +506,7 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
       goldstein_grid(d.nr_ml, d.nc_ml, &d.pi_n, &d.pj_n);
       patch_off[p] = patch_bytes;
       patch_bytes += align256(o.gold_mode == S1IFG_GOLD_FP16_SPLIT
-                                  ? (size_t)d.pi_n * band_nchunk(d.pj_n) * kGoldN * kBandCols * sizeof(float2)
+                                  ? band_partial_elems(d.pi_n, d.pj_n) * sizeof(float2)
                                   : (size_t)d.pi_n * d.pj_n * kGoldN * kGoldN * sizeof(float2));
     }
     tab_off[p] = tab_bytes;
@@ -533,7 +533,7 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
     long tab = d.grid.nrow > d.grid.ncol ? d.grid.nrow : d.grid.ncol;
     b->off[kStTables][p + 1] = b->off[kStTables][p] + tab;
     const long gitems = o.gold_mode == S1IFG_GOLD_FP16_SPLIT
-                            ? (long)d.pi_n * band_nchunk(d.pj_n)
+                            ? band_items(d.pi_n, d.pj_n)
                             : (long)d.pi_n * d.pj_n;
     b->off[kStGold][p + 1] = b->off[kStGold][p] + (d.filt ? gitems : 0);
     long gtiles = (long)((d.nc_ml + 63) / 64) * ((d.nr_ml + 3) / 4);

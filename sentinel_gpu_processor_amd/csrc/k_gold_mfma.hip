@@ -13,7 +13,7 @@ __global__ __launch_bounds__(64 * kBandWaves) __attribute__((amdgpu_waves_per_eu
   __shared__ BandLds L;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nchunk = band_nchunk(pj_n);
-  const long total = (long)pi_n * nchunk;
+  const long total = band_items(pi_n, pj_n);
   long t = (long)blockIdx.x * kBandWaves + wave;
   if (t >= total) return;
   mf::Consts K;
@@ -46,12 +46,12 @@ int device_cu_count(int device) {
 }
 
 size_t band_partial_bytes(int pi_n, int pj_n) {
-  return (size_t)pi_n * band_nchunk(pj_n) * kGoldN * kBandCols * sizeof(float2);
+  return band_partial_elems(pi_n, pj_n) * sizeof(float2);
 }
 
 int goldstein_band(s1ifg_ctx *ctx, const float2 *in, int nrow, int ncol, float alpha, int pi_n,
                    int pj_n, float2 *partial, float2 *out) {
-  const long items = (long)pi_n * band_nchunk(pj_n);
+  const long items = band_items(pi_n, pj_n);
   const long want = (items + kBandWaves - 1) / kBandWaves;
   const long cap = 2L * device_cu_count(ctx->device);
   const unsigned blocks = (unsigned)(want < cap ? want : cap);
