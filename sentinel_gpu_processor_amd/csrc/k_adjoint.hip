@@ -296,18 +296,21 @@ __global__ __launch_bounds__(kAdjBlock) void k_pair_bwd(
   if (threadIdx.x < 6) partial[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Sinc adjoint on tiles of kBwdTile x kBwdTile target pixels (one per
+// Sinc adjoint on tiles of kBwdRows x kBwdCols target pixels (one per
 // thread), the burst windows staged in LDS as in the forward tile kernel:
 // the geometry and chain partials of a pixel stay in registers while the
 // block stages the bounding box of its taps; a window larger than kBwdWin
 // samples is read from global memory instead (same arithmetic).
-constexpr int kBwdTile = 16;
-constexpr int kBwdWin = 1664;
+// 8 x 32: the target grid is rotated by the heading against the burst and
+// its rows span about two azimuth lines, so this shape needs ~1,200 samples
+// per burst (worst seen ~1,900) where 16 x 16 needed ~1,900 (~3,500).
+constexpr int kBwdRows = 8, kBwdCols = 32, kBwdThreads = kBwdRows * kBwdCols;
+constexpr int kBwdWin = 2048;
 
 struct BwdLds {
   float2 win[2][kBwdWin];
   int box[2][4];
-  double red[kBwdTile * kBwdTile / 64][6];
+  double red[kBwdThreads / 64][6];
 };
 
 __device__ __forceinline__ int wmin_i(int v) {
@@ -319,7 +322,7 @@ __device__ __forceinline__ int wmax_i(int v) {
   return v;
 }
 
-__global__ __launch_bounds__(kBwdTile * kBwdTile) void k_pair_bwd_tiles(
+__global__ __launch_bounds__(kBwdThreads) void k_pair_bwd_tiles(
     const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr,
     DemDev dem, const RowTab *rows, const ColTab *cols, const float *dh,
     const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl, int cl, int nr_ml,
@@ -330,8 +333,8 @@ __global__ __launch_bounds__(kBwdTile * kBwdTile) void k_pair_bwd_tiles(
   const float2 *Z[2] = {zr, zs};
   const OffPoly none = {{0, 0, 0, 0, 0, 0}, 0};
   const int tid = threadIdx.x;
-  const int r = blockIdx.y * kBwdTile + tid / kBwdTile;
-  const int c = blockIdx.x * kBwdTile + (tid & (kBwdTile - 1));
+  const int r = blockIdx.y * kBwdRows + tid / kBwdCols;
+  const int c = blockIdx.x * kBwdCols + (tid & (kBwdCols - 1));
   const bool active = r < nr_ml * rl && c < nc_ml * cl;
   if (tid < 8) L.box[tid >> 2][tid & 3] = (tid & 3) < 2 ? INT_MAX : INT_MIN;
   BurstAdj b[2];
@@ -386,7 +389,7 @@ __global__ __launch_bounds__(kBwdTile * kBwdTile) void k_pair_bwd_tiles(
     wa0[k] = ma - 3;
     if (staged[k]) {
       const int ns = G[k]->samples;
-      for (int row = tid >> 6; row < H; row += kBwdTile * kBwdTile / 64)
+      for (int row = tid >> 6; row < H; row += kBwdThreads / 64)
         for (int col = tid & 63; col < W; col += 64)
           L.win[k][row * W + col] = Z[k][(size_t)(wa0[k] + row) * ns + wr0[k] + col];
     }
@@ -431,7 +434,7 @@ __global__ __launch_bounds__(kBwdTile * kBwdTile) void k_pair_bwd_tiles(
   __syncthreads();
   if (tid < 6) {
     double v = 0.0;
-    for (int w = 0; w < kBwdTile * kBwdTile / 64; ++w) v += L.red[w][tid];
+    for (int w = 0; w < kBwdThreads / 64; ++w) v += L.red[w][tid];
     partial[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 6 + tid] = v;
   }
 }
@@ -625,8 +628,8 @@ int s1ifg_pair_backward(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geo
   if (nr < 1 || nc < 1) return set_error(S1IFG_EINVAL, "grid smaller than one look cell");
   long nml = (long)nr * nc;
   const bool tiles = interp == S1IFG_INTERP_SINC;
-  const dim3 tgrid((unsigned)((nc * cl + kBwdTile - 1) / kBwdTile),
-                   (unsigned)((nr * rl + kBwdTile - 1) / kBwdTile));
+  const dim3 tgrid((unsigned)((nc * cl + kBwdCols - 1) / kBwdCols),
+                   (unsigned)((nr * rl + kBwdRows - 1) / kBwdRows));
   unsigned blocks = tiles ? tgrid.x * tgrid.y : (unsigned)((nml + kAdjBlock - 1) / kAdjBlock);
   RowTab *rows;
   ColTab *cols;
@@ -647,7 +650,7 @@ int s1ifg_pair_backward(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geo
                        (const float2 *)ref_deramped, (const float2 *)sec_deramped, rl, cl, nr, nc,
                        (const float2 *)g_ifg, g_pow_ref, g_pow_sec, (double *)part, g_dh);
   else
-    hipLaunchKernelGGL(k_pair_bwd_tiles, tgrid, dim3(kBwdTile * kBwdTile), 0, ctx->stream,
+    hipLaunchKernelGGL(k_pair_bwd_tiles, tgrid, dim3(kBwdThreads), 0, ctx->stream,
                        g_ref->d_dev, g_sec->d_dev, op, gr, dd, rows, cols, dh,
                        (const float2 *)ref_deramped, (const float2 *)sec_deramped, rl, cl, nr, nc,
                        (const float2 *)g_ifg, g_pow_ref, g_pow_sec, (double *)part, g_dh);
