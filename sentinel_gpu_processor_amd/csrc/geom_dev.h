@@ -120,21 +120,31 @@ __device__ __forceinline__ int zero_doppler(const GeomDev &g, const double X[3],
 struct RowTab { double coslat, sinlat, re, fy; };
 struct ColTab { double coslon, sinlon, fx; };
 
-__device__ __forceinline__ double dem_sample(const DemDev &d, int iy, int ix) {
-  size_t o = (size_t)iy * d.ncol + ix;
-  if (d.dtype == S1IFG_DEM_I16) return (double)((const short *)d.data)[o];
-  return (double)((const float *)d.data)[o];
-}
-
-// Bilinear DEM height with indices clamped to the DEM.
+// Bilinear DEM height with indices clamped to the DEM.  The sample type is
+// tested once, so the four loads are issued together.
 __device__ __forceinline__ double dem_height(const DemDev &d, double fy, double fx) {
   fy = fmin(fmax(fy, 0.0), (double)(d.nrow - 1));
   fx = fmin(fmax(fx, 0.0), (double)(d.ncol - 1));
   int iy = min((int)floor(fy), d.nrow - 2);
   int ix = min((int)floor(fx), d.ncol - 2);
   double wy = fy - iy, wx = fx - ix;
-  double h00 = dem_sample(d, iy, ix), h01 = dem_sample(d, iy, ix + 1);
-  double h10 = dem_sample(d, iy + 1, ix), h11 = dem_sample(d, iy + 1, ix + 1);
+  const size_t o0 = (size_t)iy * d.ncol + ix, o1 = o0 + d.ncol;
+  double h00, h01, h10, h11;
+  if (d.dtype == S1IFG_DEM_I16) {
+    const short *p = (const short *)d.data;
+    const short a = p[o0], b = p[o0 + 1], c = p[o1], e = p[o1 + 1];
+    h00 = a;
+    h01 = b;
+    h10 = c;
+    h11 = e;
+  } else {
+    const float *p = (const float *)d.data;
+    const float a = p[o0], b = p[o0 + 1], c = p[o1], e = p[o1 + 1];
+    h00 = a;
+    h01 = b;
+    h10 = c;
+    h11 = e;
+  }
   return (h00 * (1 - wx) + h01 * wx) * (1 - wy) + (h10 * (1 - wx) + h11 * wx) * wy;
 }
 
