@@ -219,16 +219,20 @@ __device__ __forceinline__ void sinc_weights(float f, float w[8]) {
 // applied), the geometric range, and sin/cos of the flattening + reramp
 // phase.  Returns false outside the validity window (the phase is then not
 // computed).
-template <int INTERP>
-__device__ __forceinline__ bool geocode_geom(const GeomDev &g, const OffPoly &op,
-                                             const double X[3], double &rg, double &az,
-                                             double &rng, float &sp, float &cp) {
-  double t, dr[3];
-  zero_doppler(g, X, t, dr);
+// Sampling position of X in burst g (reproject, geo2rdr.cu:194-207): the
+// zero-Doppler time t, the geometric range rng and the range / azimuth
+// sampling positions rg / az with the offset polynomial shift (drg, daz)
+// applied.  Returns the Newton iteration count (52: not converged).
+__device__ __forceinline__ int geo_position(const GeomDev &g, const OffPoly &op,
+                                            const double X[3], double &t, double &rng,
+                                            double &rg, double &az, double &drg, double &daz) {
+  double dr[3];
+  const int it = zero_doppler(g, X, t, dr);
   rng = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
   double rgoff = (rng - g.rngstart) * g.inv_dmrg;
   double azoff = (t - g.tstart) * g.inv_dtaz;
-  double drg = 0.0, daz = 0.0;
+  drg = 0.0;
+  daz = 0.0;
   if (op.on) {
     double xs = rgoff * g.inv_samples, ys = azoff * g.inv_lines;
     drg = op.p[0] + op.p[1] * xs + op.p[2] * ys;
@@ -236,6 +240,15 @@ __device__ __forceinline__ bool geocode_geom(const GeomDev &g, const OffPoly &op
   }
   rg = rgoff + drg;
   az = azoff + daz;
+  return it;
+}
+
+template <int INTERP>
+__device__ __forceinline__ bool geocode_geom(const GeomDev &g, const OffPoly &op,
+                                             const double X[3], double &rg, double &az,
+                                             double &rng, float &sp, float &cp) {
+  double t, drg, daz;
+  geo_position(g, op, X, t, rng, rg, az, drg, daz);
   bool valid;
   if (INTERP == S1IFG_INTERP_BILINEAR)
     valid = rg > g.fvs && rg < g.lvs - 1 && az >= g.flv + 5 && az < g.llv - 5;
@@ -331,6 +344,69 @@ __device__ __forceinline__ void pixel_xyz(const GridDev &gr, const DemDev &dem,
   double h = dem_height(dem, rt.fy, ct.fx);
   if (dh) h += (double)dh[(size_t)r * gr.ncol + c];
   target_xyz(rt, ct, h, X);
+}
+
+// Whether every target pixel of rows [r0, r1] x columns [c0, c1] (DEM
+// heights, no height correction) lies outside the validity window of burst
+// g, shown from 8 points: the tile's corners at the lowest and the highest
+// DEM sample that the tile's bilinear heights blend (a blend lies between
+// its samples).  Over a tile the sampling position is an affine function of
+// (row, column, height) up to curvature terms far below a sample: the range
+// curvature over the height span is sin^2(look) dh^2 / (8 R), 0.3 m for
+// 2500 m of relief at R = 850 km, and over a ground extent of a few hundred
+// metres it is ~1e-2 m; the azimuth time's curvature is smaller still.  So
+// when all 8 points lie beyond the same edge of the window by kSkipMargin
+// samples (or lines), every pixel of the tile lies beyond it, and the
+// tile's products are the zeros that the per-pixel test gives.  Called by
+// all 64 lanes of a wavefront; the result is uniform.  A DEM box of more
+// than kSkipDemMax samples, or a corner whose Newton iteration does not
+// converge, gives false.
+constexpr double kSkipMargin = 16.0;
+constexpr int kSkipDemMax = 4096;
+
+__device__ __forceinline__ int dem_index(double f, int n) {
+  f = fmin(fmax(f, 0.0), (double)(n - 1));
+  return min((int)floor(f), n - 2);
+}
+
+template <int INTERP>
+__device__ __forceinline__ bool tile_outside(const GeomDev &g, const OffPoly &op, const DemDev &dem,
+                                             const RowTab *rows, const ColTab *cols, int r0,
+                                             int r1, int c0, int c1) {
+  const int lane = threadIdx.x & 63;
+  // DEM samples blended by the tile's pixels (the index is monotone in the
+  // row / column; one sample of slack on each side)
+  const int ya = dem_index(rows[r0].fy, dem.nrow), yb = dem_index(rows[r1].fy, dem.nrow);
+  const int xa = dem_index(cols[c0].fx, dem.ncol), xb = dem_index(cols[c1].fx, dem.ncol);
+  const int y0 = max(min(ya, yb) - 1, 0), y1 = min(max(ya, yb) + 2, dem.nrow - 1);
+  const int x0 = max(min(xa, xb) - 1, 0), x1 = min(max(xa, xb) + 2, dem.ncol - 1);
+  const int nx = x1 - x0 + 1, cnt = (y1 - y0 + 1) * nx;
+  if (cnt > kSkipDemMax) return false;
+  float lo = INFINITY, hi = -INFINITY;
+  for (int k = lane; k < cnt; k += 64) {
+    const int y = y0 + k / nx, x = x0 + k - (k / nx) * nx;
+    const size_t o = (size_t)y * dem.ncol + x;
+    const float h = dem.dtype == S1IFG_DEM_I16 ? (float)((const short *)dem.data)[o]
+                                               : ((const float *)dem.data)[o];
+    lo = fminf(lo, h);
+    hi = fmaxf(hi, h);
+  }
+  for (int m = 32; m >= 1; m >>= 1) {
+    lo = fminf(lo, __shfl_xor(lo, m));
+    hi = fmaxf(hi, __shfl_xor(hi, m));
+  }
+  // corner q = lane & 7: bit 0 column, bit 1 row, bit 2 height
+  const int q = lane & 7;
+  double X[3];
+  target_xyz(rows[(q & 2) ? r1 : r0], cols[(q & 1) ? c1 : c0], (q & 4) ? (double)hi : (double)lo,
+             X);
+  double t, rng, rg, az, drg, daz;
+  const bool ok = geo_position(g, op, X, t, rng, rg, az, drg, daz) < 52;
+  const double lr = INTERP == S1IFG_INTERP_BILINEAR ? g.fvs : g.fvs + 3;
+  const double ur = INTERP == S1IFG_INTERP_BILINEAR ? g.lvs - 1 : g.lvs - 4;
+  const double la = g.flv + 5, ua = g.llv - 5, M = kSkipMargin;
+  return __all(ok && rg < lr - M) || __all(ok && rg >= ur + M) || __all(ok && az < la - M) ||
+         __all(ok && az >= ua + M);
 }
 
 // deramp_burst element (geo2rdr.cu:138-158): z * exp(i phi),

@@ -1,7 +1,11 @@
 // Fused burst pair with sinc interpolation, burst windows staged in LDS
 // (shared by the per-pair and batched kernels).
 //
-// A 256-thread block owns a tile of 16 x 16 look cells.  Each thread owns
+// A 256-thread block owns a tile of 16 x 16 look cells.  Two wavefronts
+// first test whether the whole tile lies outside either burst
+// (tile_outside: about half of a burst's target grid, the corners of its
+// latitude / longitude box, does); a burst that the tile misses is not
+// geocoded, and a tile that misses both writes zeros.  Each thread owns
 // one cell (rl * cl <= kPairMaxPix target pixels): it first computes the
 // FP64 geometry and phase of its pixels in both bursts and keeps the
 // integer / fractional sampling positions and sin/cos of the phase in
@@ -28,6 +32,7 @@ constexpr int kPairMaxPix = 4;       // target pixels per look cell on this path
 struct PairLds {
   float2 win[2][kPairWin];
   int box[2][4];  // per burst: min intr, min inta, max intr, max inta
+  int out[2];     // per burst: every pixel of the tile outside the burst
 };
 
 __device__ __forceinline__ int wave_min_i(int v) {
@@ -59,6 +64,31 @@ __device__ __forceinline__ void pair_tile_sinc(
   int lo_r[2] = {INT_MAX, INT_MAX}, lo_a[2] = {INT_MAX, INT_MAX};
   int hi_r[2] = {INT_MIN, INT_MIN}, hi_a[2] = {INT_MIN, INT_MIN};
   if (tid < 8) L.box[tid >> 2][tid & 3] = (tid & 3) < 2 ? INT_MAX : INT_MIN;
+  // wavefront b < 2: is the whole tile outside burst b?
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (wave < 2) {
+    bool o = false;
+    if (!dh) {
+      const int nrt = min(nr_ml - tile_y * kPairTile, kPairTile);
+      const int nct = min(nc_ml - tile_x * kPairTile, kPairTile);
+      const int r0 = tile_y * kPairTile * rl, c0 = tile_x * kPairTile * cl;
+      o = tile_outside<S1IFG_INTERP_SINC>(wave ? G1 : G0, wave ? op1 : none, dem, rows, cols, r0,
+                                          r0 + nrt * rl - 1, c0, c0 + nct * cl - 1);
+    }
+    if ((tid & 63) == 0) L.out[wave] = o;
+  }
+  __syncthreads();  // box initialised, L.out set
+  const bool out0 = L.out[0], out1 = L.out[1];
+  if (out0 && out1) {
+    // no pixel of the tile in either burst: the products are zero
+    if (active) {
+      size_t o = (size_t)rm * nc_ml + cm;
+      ifg[o] = make_float2(0.f, 0.f);
+      if (pr) pr[o] = 0.f;
+      if (ps) ps[o] = 0.f;
+    }
+    return;
+  }
   // geometry of the cell's pixels (rows outer, columns inner)
 #pragma unroll
   for (int p = 0; p < kPairMaxPix; ++p) {
@@ -73,7 +103,8 @@ __device__ __forceinline__ void pair_tile_sinc(
       for (int b = 0; b < 2; ++b) {
         double rg, az, rng;
         float s, c;
-        if (geocode_geom<S1IFG_INTERP_SINC>(b ? G1 : G0, b ? op1 : none, X, rg, az, rng, s, c)) {
+        if (!(b ? out1 : out0) &&
+            geocode_geom<S1IFG_INTERP_SINC>(b ? G1 : G0, b ? op1 : none, X, rg, az, rng, s, c)) {
           const int r0 = (int)floor(rg), a0 = (int)floor(az);
           ir[b][p] = r0;
           ia[b][p] = a0;
@@ -90,7 +121,6 @@ __device__ __forceinline__ void pair_tile_sinc(
       }
     }
   }
-  __syncthreads();  // box initialised
 #pragma unroll
   for (int b = 0; b < 2; ++b) {
     const int v0 = wave_min_i(lo_r[b]), v1 = wave_min_i(lo_a[b]);
