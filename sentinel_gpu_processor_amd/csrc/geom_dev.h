@@ -346,11 +346,12 @@ __device__ __forceinline__ void pixel_xyz(const GridDev &gr, const DemDev &dem,
   target_xyz(rt, ct, h, X);
 }
 
-// Whether every target pixel of rows [r0, r1] x columns [c0, c1] (DEM
-// heights, no height correction) lies outside the validity window of burst
-// g, shown from 8 points: the tile's corners at the lowest and the highest
-// DEM sample that the tile's bilinear heights blend (a blend lies between
-// its samples).  Over a tile the sampling position is an affine function of
+// Whether every target pixel of rows [r0, r1] x columns [c0, c1] lies
+// outside the validity window of burst g, shown from 8 points: the tile's
+// corners at the lowest and the highest height of its pixels, bounded by
+// the extreme DEM samples that the tile's bilinear heights blend (a blend
+// lies between its samples) plus the extremes of the height correction dh
+// (row stride ld) over the tile when there is one.  Over a tile the sampling position is an affine function of
 // (row, column, height) up to curvature terms far below a sample: the range
 // curvature over the height span is sin^2(look) dh^2 / (8 R), 0.3 m for
 // 2500 m of relief at R = 850 km, and over a ground extent of a few hundred
@@ -358,9 +359,9 @@ __device__ __forceinline__ void pixel_xyz(const GridDev &gr, const DemDev &dem,
 // when all 8 points lie beyond the same edge of the window by kSkipMargin
 // samples (or lines), every pixel of the tile lies beyond it, and the
 // tile's products are the zeros that the per-pixel test gives.  Called by
-// all 64 lanes of a wavefront; the result is uniform.  A DEM box of more
-// than kSkipDemMax samples, or a corner whose Newton iteration does not
-// converge, gives false.
+// all 64 lanes of a wavefront; the result is uniform.  A DEM box or a tile
+// of more than kSkipDemMax samples, or a corner whose Newton iteration does
+// not converge, gives false.
 constexpr double kSkipMargin = 16.0;
 constexpr int kSkipDemMax = 4096;
 
@@ -371,8 +372,9 @@ __device__ __forceinline__ int dem_index(double f, int n) {
 
 template <int INTERP>
 __device__ __forceinline__ bool tile_outside(const GeomDev &g, const OffPoly &op, const DemDev &dem,
-                                             const RowTab *rows, const ColTab *cols, int r0,
-                                             int r1, int c0, int c1) {
+                                             const RowTab *rows, const ColTab *cols,
+                                             const float *dh, int ld, int r0, int r1, int c0,
+                                             int c1) {
   const int lane = threadIdx.x & 63;
   // DEM samples blended by the tile's pixels (the index is monotone in the
   // row / column; one sample of slack on each side)
@@ -391,15 +393,30 @@ __device__ __forceinline__ bool tile_outside(const GeomDev &g, const OffPoly &op
     lo = fminf(lo, h);
     hi = fmaxf(hi, h);
   }
+  float dlo = 0.f, dhi = 0.f;
+  if (dh) {
+    const int tw = c1 - c0 + 1, tn = (r1 - r0 + 1) * tw;
+    if (tn > kSkipDemMax) return false;
+    dlo = INFINITY;
+    dhi = -INFINITY;
+    for (int k = lane; k < tn; k += 64) {
+      const int y = r0 + k / tw, x = c0 + k - (k / tw) * tw;
+      const float v = dh[(size_t)y * ld + x];
+      dlo = fminf(dlo, v);
+      dhi = fmaxf(dhi, v);
+    }
+  }
   for (int m = 32; m >= 1; m >>= 1) {
     lo = fminf(lo, __shfl_xor(lo, m));
     hi = fmaxf(hi, __shfl_xor(hi, m));
+    dlo = fminf(dlo, __shfl_xor(dlo, m));
+    dhi = fmaxf(dhi, __shfl_xor(dhi, m));
   }
   // corner q = lane & 7: bit 0 column, bit 1 row, bit 2 height
   const int q = lane & 7;
   double X[3];
-  target_xyz(rows[(q & 2) ? r1 : r0], cols[(q & 1) ? c1 : c0], (q & 4) ? (double)hi : (double)lo,
-             X);
+  target_xyz(rows[(q & 2) ? r1 : r0], cols[(q & 1) ? c1 : c0],
+             (q & 4) ? (double)hi + (double)dhi : (double)lo + (double)dlo, X);
   double t, rng, rg, az, drg, daz;
   const bool ok = geo_position(g, op, X, t, rng, rg, az, drg, daz) < 52;
   const double lr = INTERP == S1IFG_INTERP_BILINEAR ? g.fvs : g.fvs + 3;

@@ -310,6 +310,7 @@ constexpr int kBwdWin = 2048;
 struct BwdLds {
   float2 win[2][kBwdWin];
   int box[2][4];
+  int out[2];  // per burst: every pixel of the tile outside the burst
   double red[kBwdThreads / 64][6];
 };
 
@@ -337,12 +338,30 @@ __global__ __launch_bounds__(kBwdThreads) void k_pair_bwd_tiles(
   const int c = blockIdx.x * kBwdCols + (tid & (kBwdCols - 1));
   const bool active = r < nr_ml * rl && c < nc_ml * cl;
   if (tid < 8) L.box[tid >> 2][tid & 3] = (tid & 3) < 2 ? INT_MAX : INT_MIN;
+  // wavefront k < 2: is the whole tile outside burst k?  (tile_outside)
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (wave < 2) {
+    const int r0 = blockIdx.y * kBwdRows, c0 = blockIdx.x * kBwdCols;
+    const int r1 = min(r0 + kBwdRows, nr_ml * rl) - 1, c1 = min(c0 + kBwdCols, nc_ml * cl) - 1;
+    const bool o = tile_outside<S1IFG_INTERP_SINC>(wave ? *gps : *gpr, wave ? ops : none, dem,
+                                                   rows, cols, dh, gr.ncol, r0, r1, c0, c1);
+    if ((tid & 63) == 0) L.out[wave] = o;
+  }
+  __syncthreads();  // box initialised, L.out set
+  const bool out[2] = {L.out[0] != 0, L.out[1] != 0};
+  if (out[0] && out[1]) {
+    // no pixel of the tile in either burst: zero gradients
+    if (active && g_dh) g_dh[(size_t)r * gr.ncol + c] = 0.f;
+    if (tid < 6) partial[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 6 + tid] = 0.0;
+    return;
+  }
   BurstAdj b[2];
   int ir[2] = {0, 0}, ia[2] = {0, 0};
   float fr[2] = {0.f, 0.f}, fa[2] = {0.f, 0.f}, sp[2] = {0.f, 0.f}, cp[2] = {1.f, 1.f};
   int lo_r[2] = {INT_MAX, INT_MAX}, lo_a[2] = {INT_MAX, INT_MAX};
   int hi_r[2] = {INT_MIN, INT_MIN}, hi_a[2] = {INT_MIN, INT_MIN};
   b[0].valid = b[1].valid = false;
+  b[0].z = b[1].z = make_float2(0.f, 0.f);
   if (active) {
     double X[3];
     pixel_xyz(gr, dem, rows, cols, dh, r, c, X);
@@ -352,8 +371,8 @@ __global__ __launch_bounds__(kBwdThreads) void k_pair_bwd_tiles(
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       double rg, az;
-      if (burst_geom_adj<S1IFG_INTERP_SINC>(*G[k], k ? ops : none, X, nrm, rg, az, sp[k], cp[k],
-                                            b[k])) {
+      if (!out[k] && burst_geom_adj<S1IFG_INTERP_SINC>(*G[k], k ? ops : none, X, nrm, rg, az,
+                                                       sp[k], cp[k], b[k])) {
         ir[k] = (int)floor(rg);
         ia[k] = (int)floor(az);
         fr[k] = (float)(rg - ir[k]);
@@ -363,7 +382,6 @@ __global__ __launch_bounds__(kBwdThreads) void k_pair_bwd_tiles(
       }
     }
   }
-  __syncthreads();  // box initialised
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int v0 = wmin_i(lo_r[k]), v1 = wmin_i(lo_a[k]);

@@ -67,14 +67,12 @@ __device__ __forceinline__ void pair_tile_sinc(
   // wavefront b < 2: is the whole tile outside burst b?
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (wave < 2) {
-    bool o = false;
-    if (!dh) {
-      const int nrt = min(nr_ml - tile_y * kPairTile, kPairTile);
-      const int nct = min(nc_ml - tile_x * kPairTile, kPairTile);
-      const int r0 = tile_y * kPairTile * rl, c0 = tile_x * kPairTile * cl;
-      o = tile_outside<S1IFG_INTERP_SINC>(wave ? G1 : G0, wave ? op1 : none, dem, rows, cols, r0,
-                                          r0 + nrt * rl - 1, c0, c0 + nct * cl - 1);
-    }
+    const int nrt = min(nr_ml - tile_y * kPairTile, kPairTile);
+    const int nct = min(nc_ml - tile_x * kPairTile, kPairTile);
+    const int r0 = tile_y * kPairTile * rl, c0 = tile_x * kPairTile * cl;
+    const bool o = tile_outside<S1IFG_INTERP_SINC>(wave ? G1 : G0, wave ? op1 : none, dem, rows,
+                                                   cols, dh, gr.ncol, r0, r0 + nrt * rl - 1, c0,
+                                                   c0 + nct * cl - 1);
     if ((tid & 63) == 0) L.out[wave] = o;
   }
   __syncthreads();  // box initialised, L.out set
