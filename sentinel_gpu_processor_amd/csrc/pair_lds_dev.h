@@ -26,11 +26,11 @@
 namespace s1ifg {
 
 constexpr int kPairTile = 16;        // look cells per tile side
-constexpr int kPairWin = 2432;       // staged samples per burst (19 KB; 4 blocks per CU)
+constexpr int kPairWin = 4864;       // staged samples of burst 0 (38 KB; 4 blocks per CU)
 constexpr int kPairMaxPix = 4;       // target pixels per look cell on this path
 
 struct PairLds {
-  float2 win[2][kPairWin];
+  float2 win[1][kPairWin];
   int box[2][4];  // per burst: min intr, min inta, max intr, max inta
   int out[2];     // per burst: every pixel of the tile outside the burst
 };
@@ -139,7 +139,9 @@ __device__ __forceinline__ void pair_tile_sinc(
     const int mr = L.box[b][0], ma = L.box[b][1], xr = L.box[b][2], xa = L.box[b][3];
     const bool any = mr <= xr;
     const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
-    staged[b] = any && W * H <= kPairWin;
+    // burst 0 from LDS, burst 1 by gathers through the vector L1: the
+    // two memory paths run side by side
+    staged[b] = b == 0 && any && W * H <= kPairWin;
     wc[b] = (int)W;
     wr0[b] = mr - 3;
     wa0[b] = ma - 3;
@@ -148,7 +150,7 @@ __device__ __forceinline__ void pair_tile_sinc(
       const float2 *z = b ? z1 : z0;
       for (int row = tid >> 6; row < H; row += 4)
         for (int col = tid & 63; col < W; col += 64)
-          L.win[b][row * W + col] = z[(size_t)(wa0[b] + row) * g.samples + wr0[b] + col];
+          L.win[0][row * W + col] = z[(size_t)(wa0[b] + row) * g.samples + wr0[b] + col];
     }
   }
   __syncthreads();
@@ -165,7 +167,7 @@ __device__ __forceinline__ void pair_tile_sinc(
         v[b] = make_float2(0.f, 0.f);
         if (vmask & (1u << (2 * p + b))) {
           if (staged[b]) {
-            v[b] = sinc_interp(&L.win[b][(ia[b][p] - 3 - wa0[b]) * wc[b] + (ir[b][p] - 3 - wr0[b])],
+            v[b] = sinc_interp(&L.win[0][(ia[b][p] - 3 - wa0[b]) * wc[b] + (ir[b][p] - 3 - wr0[b])],
                                wc[b], fr[b][p], fa[b][p], sn[b][p], cs[b][p]);
           } else {
             const GeomDev &g = b ? G1 : G0;
