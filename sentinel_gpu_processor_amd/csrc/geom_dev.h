@@ -180,16 +180,22 @@ __device__ __forceinline__ void sincos_phase(double ph, float &s, float &c) {
 // With sin(pi (f - k)) = (-1)^k sin(pi f), sinc(f - k) is
 // (-1)^k sin(pi f) / (pi P) * prod_{j != k} (f - j), P = prod_j (f - j); the
 // factor sin(pi f) / (pi P) is common to all taps and cancels in the
-// normalisation, so w_k = q_k / sum_j q_j with
-// q_k = (-1)^k cos^2(pi (f - k)/8) prod_{j != k} (f - j) (prefix and suffix
-// products; no per-tap reciprocal, and f = 0 gives w = delta_k0 directly).
-__device__ __forceinline__ void sinc_weights(float f, float w[8]) {
+// normalisation, and so does a constant factor of the window
+// (2 cos^2(x/2) = 1 + cos x).  So w_k = q_k / sum_j q_j with
+// q_k = (-1)^k (1 + cos(pi (f - k)/4)) prod_{j != k} (f - j) (prefix and
+// suffix products; f = 0 gives q = delta_k0 times a constant).  Writes the
+// q_k and returns 1 / sum_j q_j: the caller applies the normalisation once
+// to its result.
+__device__ __forceinline__ float sinc_weights(float f, float q[8]) {
   // hardware sin/cos take revolutions: sin(pi f / 4) = sin(2 pi (f/8))
   const float s4 = __builtin_amdgcn_sinf(0.125f * f);
   const float c4 = __builtin_amdgcn_cosf(0.125f * f);
-  // cos(pi k/4), sin(pi k/4) for k = -3..4
-  const float ck[8] = {-0.70710678f, 0.0f, 0.70710678f, 1.0f, 0.70710678f, 0.0f, -0.70710678f, -1.0f};
-  const float sk[8] = {-0.70710678f, -1.0f, -0.70710678f, 0.0f, 0.70710678f, 1.0f, 0.70710678f, 0.0f};
+  // cos(pi (f - k)/4) = c4 cos(pi k/4) + s4 sin(pi k/4), k = -3..4:
+  // -u, -s4, v, c4, u, s4, -v, -c4 with u, v = (c4 +- s4) / sqrt(2)
+  const float u = 0.70710678f * (c4 + s4), v = 0.70710678f * (c4 - s4);
+  // (-1)^k (1 + cos(pi (f - k)/4))
+  const float win[8] = {u - 1.0f,  1.0f - s4, -1.0f - v, 1.0f + c4,
+                        -1.0f - u, 1.0f + s4, v - 1.0f,  1.0f - c4};
   float d[8], pre[8], suf[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) d[i] = f - (float)(i - 3);
@@ -202,16 +208,10 @@ __device__ __forceinline__ void sinc_weights(float f, float w[8]) {
   float sum = 0.f;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    // cos(pi (f-k)/4) = cos(pi f/4) cos(pi k/4) + sin(pi f/4) sin(pi k/4)
-    const float cw = c4 * ck[i] + s4 * sk[i];
-    const float win = 0.5f * (1.0f + cw);
-    const float q = pre[i] * suf[i] * win;
-    w[i] = (i & 1) ? q : -q;  // (-1)^k, k = i - 3
-    sum += w[i];
+    q[i] = pre[i] * suf[i] * win[i];
+    sum += q[i];
   }
-  const float inv = __builtin_amdgcn_rcpf(sum);
-#pragma unroll
-  for (int i = 0; i < 8; ++i) w[i] *= inv;
+  return __builtin_amdgcn_rcpf(sum);
 }
 
 // Geometry and phase of one target pixel of one burst (reproject,
@@ -257,9 +257,10 @@ __device__ __forceinline__ bool geocode_geom(const GeomDev &g, const OffPoly &op
   if (!valid) return false;
   // reramp (geo2rdr.cu:210-219) at the sampling position
   double rp = rng + drg * g.dmrg;
-  double dt = rp * 2 / kSol - g.fm[0];
+  const double trp = rp * (2.0 / kSol);  // two-way range time
+  double dt = trp - g.fm[0];
   double ka = g.fm[1] + dt * (g.fm[2] + g.fm[3] * dt);
-  dt = rp * 2 / kSol - g.dc[0];
+  dt = trp - g.dc[0];
   double fnc = g.dc[1] + dt * (g.dc[2] + g.dc[3] * dt);
   double kt = ka * g.ks * rcp64(ka - g.ks);
   double etac = -fnc * rcp64(ka);
@@ -267,18 +268,19 @@ __device__ __forceinline__ bool geocode_geom(const GeomDev &g, const OffPoly &op
   double eta = -g.lines * g.dtaz / 2. + (t - g.tstart) + daz * g.dtaz;
   double etadiff = eta - etaref;
   double reramp = -kPi * kt * etadiff * etadiff;
-  sincos_phase(4.0 * kPi / g.wvl * rng - reramp, sp, cp);
+  sincos_phase(g.k4pi_wvl * rng - reramp, sp, cp);
   return true;
 }
 
 // 8 x 8 sinc interpolation of the deramped burst at fractional offsets
 // (fr, fa) from the window whose top-left tap is base (row stride ld),
-// rotated by (cp, sp).
+// rotated by (cp, sp).  The taps are weighted with the unnormalised q of
+// sinc_weights; both normalisations scale the rotation.
 __device__ __forceinline__ float2 sinc_interp(const float2 *__restrict__ base, long ld, float fr,
                                               float fa, float sp, float cp) {
   float wr[8], wa[8];
-  sinc_weights(fr, wr);
-  sinc_weights(fa, wa);
+  const float nr = sinc_weights(fr, wr);
+  const float na = sinc_weights(fa, wa);
   float rx = 0.f, ry = 0.f;
 #pragma unroll
   for (int ia = 0; ia < 8; ++ia) {
@@ -293,7 +295,9 @@ __device__ __forceinline__ float2 sinc_interp(const float2 *__restrict__ base, l
     rx = fmaf(wa[ia], ax, rx);
     ry = fmaf(wa[ia], ay, ry);
   }
-  return make_float2(rx * cp - ry * sp, rx * sp + ry * cp);
+  // normalisation of both weight sets, applied with the rotation
+  const float sc = nr * na, c = cp * sc, s = sp * sc;
+  return make_float2(rx * c - ry * s, rx * s + ry * c);
 }
 
 // Geocode one target pixel of one burst (reproject, geo2rdr.cu:194-247).

@@ -410,6 +410,7 @@ int s1ifg_geom_create(s1ifg_ctx *ctx, const s1ifg_orbit *orbit, const s1ifg_burs
     for (int m = 0; m < 24; ++m) recs[(size_t)(k - 1) * 32 + 1 + m] = coef[(size_t)(k - 1) * 24 + m];
   }
   gd.inv_dmrg = 1.0 / gd.dmrg;
+  gd.k4pi_wvl = 4.0 * kPi / gd.wvl;
   gd.inv_dtaz = b->prf;
   gd.inv_samples = 1.0 / N;
   gd.inv_lines = 1.0 / L;

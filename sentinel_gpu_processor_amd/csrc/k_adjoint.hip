@@ -132,7 +132,7 @@ __device__ __forceinline__ bool burst_geom_adj(const GeomDev &g, const OffPoly &
   double ed = eta - etaref;
   o.R_rp = -kPi * (dkt * ed * ed - 2.0 * kt * ed * detaref);
   o.R_eta = -2.0 * kPi * kt * ed;
-  sincos_phase(4.0 * kPi / g.wvl * rng + kPi * kt * ed * ed, sp, cp);
+  sincos_phase(g.k4pi_wvl * rng + kPi * kt * ed * ed, sp, cp);
   // height partials
   double vn = v[0] * nrm[0] + v[1] * nrm[1] + v[2] * nrm[2];
   double den = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] - (dr[0] * a[0] + dr[1] * a[1] + dr[2] * a[2]);
@@ -231,7 +231,7 @@ __device__ __forceinline__ double burst_chain(const GeomDev &g, const OffPoly &o
     g_rgoff += (gDRG * op.p[1] + gDAZ * op.p[4]) * g.inv_samples;
     g_azoff += (gDRG * op.p[2] + gDAZ * op.p[5]) * g.inv_lines;
   }
-  double g_rng = gPHI * (4.0 * kPi / g.wvl) + g_rgoff * g.inv_dmrg + gRP;
+  double g_rng = gPHI * g.k4pi_wvl + g_rgoff * g.inv_dmrg + gRP;
   double g_t = g_azoff * g.inv_dtaz + gETA;
   return g_t * b.dt_dh + g_rng * b.drng_dh;
 }

@@ -52,6 +52,7 @@ struct GeomDev {
   double tmid, xmid[3], vmid[3], amid[3];
   double ati;    // azimuth_time_interval (deramp eta)
   double inv_dmrg, inv_dtaz, inv_samples, inv_lines;
+  double k4pi_wvl;  // 4 pi / wvl (two-way phase per metre of range)
   int lines, samples;
   int flv, llv, fvs, lvs;
   const double *kt;      // [samples]
