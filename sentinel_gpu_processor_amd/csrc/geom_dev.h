@@ -69,6 +69,28 @@ __device__ __forceinline__ double rcp64(double x) {
   return fma(r, e, r);
 }
 
+// One Newton iteration of zero_doppler from the orbit state (x, v, a):
+// d = X - x and the time step dt.
+__device__ __forceinline__ void zd_step(const double X[3], const double x[3], const double v[3],
+                                        const double a[3], double d[3], double &dt) {
+  d[0] = X[0] - x[0];
+  d[1] = X[1] - x[1];
+  d[2] = X[2] - x[2];
+  double fn = d[0] * v[0] + d[1] * v[1] + d[2] * v[2];
+  double fp = -(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + (d[0] * a[0] + d[1] * a[1] + d[2] * a[2]);
+  dt = -fn * rcp64(fp);
+}
+
+// dr = X - x(t + dt) after the converged step, by the second-order Taylor
+// step of the orbit.
+__device__ __forceinline__ void zd_finish(const double d[3], const double v[3], const double a[3],
+                                          double dt, double dr[3]) {
+  double h2 = 0.5 * dt * dt;
+  dr[0] = d[0] - v[0] * dt - a[0] * h2;
+  dr[1] = d[1] - v[1] * dt - a[1] * h2;
+  dr[2] = d[2] - v[2] * dt - a[2] * h2;
+}
+
 // Zero-Doppler azimuth time of X (orbitrangetime, orbit.cu:247-279).  The
 // iteration starts from the burst mid-time state as the reference does
 // (geo2rdr.cu:588-600).  The Newton derivative includes the dr.a term, so
@@ -84,16 +106,11 @@ __device__ __forceinline__ int zero_doppler(const GeomDev &g, const double X[3],
   double a[3] = {g.amid[0], g.amid[1], g.amid[2]};
   t = g.tmid;
   for (int it = 0; it < 51; ++it) {
-    double d0 = X[0] - x[0], d1 = X[1] - x[1], d2 = X[2] - x[2];
-    double fn = d0 * v[0] + d1 * v[1] + d2 * v[2];
-    double fp = -(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + (d0 * a[0] + d1 * a[1] + d2 * a[2]);
-    double dt = -fn * rcp64(fp);
+    double d[3], dt;
+    zd_step(X, x, v, a, d, dt);
     t += dt;
     if (fabs(dt) < 5.0e-9) {
-      double h2 = 0.5 * dt * dt;
-      dr[0] = d0 - v[0] * dt - a[0] * h2;
-      dr[1] = d1 - v[1] * dt - a[1] * h2;
-      dr[2] = d2 - v[2] * dt - a[2] * h2;
+      zd_finish(d, v, a, dt, dr);
       if (vout) {
         for (int k = 0; k < 3; ++k) {
           vout[k] = v[k] + a[k] * dt;
@@ -214,20 +231,13 @@ __device__ __forceinline__ float sinc_weights(float f, float q[8]) {
   return __builtin_amdgcn_rcpf(sum);
 }
 
-// Geometry and phase of one target pixel of one burst (reproject,
-// geo2rdr.cu:194-219): the sampling position rg/az (offset polynomial
-// applied), the geometric range, and sin/cos of the flattening + reramp
-// phase.  Returns false outside the validity window (the phase is then not
-// computed).
-// Sampling position of X in burst g (reproject, geo2rdr.cu:194-207): the
-// zero-Doppler time t, the geometric range rng and the range / azimuth
-// sampling positions rg / az with the offset polynomial shift (drg, daz)
-// applied.  Returns the Newton iteration count (52: not converged).
-__device__ __forceinline__ int geo_position(const GeomDev &g, const OffPoly &op,
-                                            const double X[3], double &t, double &rng,
-                                            double &rg, double &az, double &drg, double &daz) {
-  double dr[3];
-  const int it = zero_doppler(g, X, t, dr);
+// Sampling position in burst g of the target at zero-Doppler time t and
+// offset dr from the orbit (reproject, geo2rdr.cu:200-207): the geometric
+// range rng and the range / azimuth sampling positions rg / az with the
+// offset polynomial shift (drg, daz) applied.
+__device__ __forceinline__ void sample_position(const GeomDev &g, const OffPoly &op, double t,
+                                                const double dr[3], double &rng, double &rg,
+                                                double &az, double &drg, double &daz) {
   rng = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
   double rgoff = (rng - g.rngstart) * g.inv_dmrg;
   double azoff = (t - g.tstart) * g.inv_dtaz;
@@ -240,22 +250,33 @@ __device__ __forceinline__ int geo_position(const GeomDev &g, const OffPoly &op,
   }
   rg = rgoff + drg;
   az = azoff + daz;
+}
+
+// Sampling position of X in burst g (reproject, geo2rdr.cu:194-207): the
+// zero-Doppler time t, then sample_position.  Returns the Newton iteration
+// count (52: not converged).
+__device__ __forceinline__ int geo_position(const GeomDev &g, const OffPoly &op,
+                                            const double X[3], double &t, double &rng,
+                                            double &rg, double &az, double &drg, double &daz) {
+  double dr[3];
+  const int it = zero_doppler(g, X, t, dr);
+  sample_position(g, op, t, dr, rng, rg, az, drg, daz);
   return it;
 }
 
+// Validity window of the interpolation (geo2rdr.cu:208-209 for bilinear;
+// the 8-tap window for sinc).
 template <int INTERP>
-__device__ __forceinline__ bool geocode_geom(const GeomDev &g, const OffPoly &op,
-                                             const double X[3], double &rg, double &az,
-                                             double &rng, float &sp, float &cp) {
-  double t, drg, daz;
-  geo_position(g, op, X, t, rng, rg, az, drg, daz);
-  bool valid;
+__device__ __forceinline__ bool sample_valid(const GeomDev &g, double rg, double az) {
   if (INTERP == S1IFG_INTERP_BILINEAR)
-    valid = rg > g.fvs && rg < g.lvs - 1 && az >= g.flv + 5 && az < g.llv - 5;
-  else
-    valid = rg >= g.fvs + 3 && rg < g.lvs - 4 && az >= g.flv + 5 && az < g.llv - 5;
-  if (!valid) return false;
-  // reramp (geo2rdr.cu:210-219) at the sampling position
+    return rg > g.fvs && rg < g.lvs - 1 && az >= g.flv + 5 && az < g.llv - 5;
+  return rg >= g.fvs + 3 && rg < g.lvs - 4 && az >= g.flv + 5 && az < g.llv - 5;
+}
+
+// sin / cos of the flattening + reramp phase (geo2rdr.cu:210-219) at the
+// sampling position.
+__device__ __forceinline__ void reramp_phase(const GeomDev &g, double rng, double drg, double daz,
+                                             double t, float &sp, float &cp) {
   double rp = rng + drg * g.dmrg;
   const double trp = rp * (2.0 / kSol);  // two-way range time
   double dt = trp - g.fm[0];
@@ -269,6 +290,21 @@ __device__ __forceinline__ bool geocode_geom(const GeomDev &g, const OffPoly &op
   double etadiff = eta - etaref;
   double reramp = -kPi * kt * etadiff * etadiff;
   sincos_phase(g.k4pi_wvl * rng - reramp, sp, cp);
+}
+
+// Geometry and phase of one target pixel of one burst (reproject,
+// geo2rdr.cu:194-219): the sampling position rg/az (offset polynomial
+// applied), the geometric range, and sin/cos of the flattening + reramp
+// phase.  Returns false outside the validity window (the phase is then not
+// computed).
+template <int INTERP>
+__device__ __forceinline__ bool geocode_geom(const GeomDev &g, const OffPoly &op,
+                                             const double X[3], double &rg, double &az,
+                                             double &rng, float &sp, float &cp) {
+  double t, drg, daz;
+  geo_position(g, op, X, t, rng, rg, az, drg, daz);
+  if (!sample_valid<INTERP>(g, rg, az)) return false;
+  reramp_phase(g, rng, drg, daz, t, sp, cp);
   return true;
 }
 
