@@ -59,6 +59,48 @@ __device__ __forceinline__ void orbit_eval(const OrbitDev &o, double t, double x
   }
 }
 
+// Orbit state at t from the node record rec (any address space): the
+// cubic Taylor step of the node's stencil polynomial.
+template <typename P>
+__device__ __forceinline__ void orbit_node_step(P rec, double t, double x[3], double v[3],
+                                                double a[3]) {
+  const double e = t - rec[0];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double xc = rec[1 + c], vc = rec[4 + c], ac = rec[7 + c], jc = rec[10 + c];
+    x[c] = fma(e, fma(e, fma(e, jc * (1.0 / 6.0), 0.5 * ac), vc), xc);
+    v[c] = fma(e, fma(e, 0.5 * jc, ac), vc);
+    a[c] = fma(e, jc, ac);
+  }
+}
+
+// orbit_eval from the orbit node table: the stencil index as orbit_eval,
+// then the nearest of the node_m + 1 nodes of that stencil interval and a
+// cubic Taylor step, within 1e-10 m and 1e-10 m/s of orbit_eval (the node
+// count is chosen on that bound when the geometry is built).  Scalar loads
+// when the wavefront shares the node.  Without a node table, or for a time
+// outside the interior stencil intervals, orbit_eval.
+__device__ __forceinline__ void orbit_eval_fast(const OrbitDev &o, double t, double x[3],
+                                                double v[3], double a[3]) {
+  const double u = (t - o.t0) * o.inv_h;
+  int k = (int)u;
+  k = k > 1 ? k : 1;
+  k = k < o.n - 3 ? k : o.n - 3;
+  const double f = u - k;  // position in stencil interval k
+  if (!__all(o.node_m > 0 && f >= 0.0 && f <= 1.0)) {
+    orbit_eval(o, t, x, v, a);
+    return;
+  }
+  const int idx = (k - 1) * (o.node_m + 1) + (int)rint(f * o.node_m);
+  const int i0 = __builtin_amdgcn_readfirstlane(idx);
+  if (__all(idx == i0)) {
+    typedef const __attribute__((address_space(4))) double *CP;
+    orbit_node_step((CP)o.node + (size_t)i0 * kNodeRec, t, x, v, a);
+  } else {
+    orbit_node_step(o.node + (size_t)idx * kNodeRec, t, x, v, a);
+  }
+}
+
 // 1/x in FP64 from the float reciprocal and two Newton steps (relative
 // error ~1e-16 for |x| within float range, x != 0).
 __device__ __forceinline__ double rcp64(double x) {
@@ -119,7 +161,7 @@ __device__ __forceinline__ int zero_doppler(const GeomDev &g, const double X[3],
       }
       return it + 1;
     }
-    orbit_eval(g.orb, t, x, v, a);
+    orbit_eval_fast(g.orb, t, x, v, a);
   }
   dr[0] = X[0] - x[0];
   dr[1] = X[1] - x[1];

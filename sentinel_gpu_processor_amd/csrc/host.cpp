@@ -189,6 +189,48 @@ static void hermite_poly(const double s[4], const double x[4][3], const double v
   }
 }
 
+// Taylor coefficients of p(s0 + e) = sum_j c[j] e^j from the ascending
+// power coefficients a[0..7] of p(s).
+static void poly_shift(const double *a, double s0, double c[8]) {
+  for (int j = 0; j < 8; ++j) {
+    double sum = 0, binom = 1, pw = 1;  // C(i, j) s0^(i-j) for i = j, j+1, ...
+    for (int i = j; i < 8; ++i) {
+      sum += a[i] * binom * pw;
+      binom = binom * (i + 1) / (i + 1 - j);
+      pw *= s0;
+    }
+    c[j] = sum;
+  }
+}
+
+// Node intervals per stencil interval for the orbit node table: the
+// smallest M (4 .. 256) at which a cubic Taylor step from the nearest of
+// the M + 1 nodes of an interval, |e| <= 1 / (2 M) in interval units,
+// reproduces every interval polynomial to within 1e-10 m, 1e-10 m/s and
+// 1e-8 m/s^2 (the sum of the omitted terms bounds the difference); 0 when
+// none does.  nst intervals of coef[24] (ascending powers per component).
+static int orbit_node_count(const std::vector<double> &coef, int nst, double h) {
+  for (int M = 4; M <= 256; M *= 2) {
+    const double e = 0.5 / M;
+    bool ok = true;
+    for (int k = 0; k < nst && ok; ++k)
+      for (int m = 0; m <= M && ok; ++m)
+        for (int comp = 0; comp < 3 && ok; ++comp) {
+          double c[8];
+          poly_shift(&coef[(size_t)k * 24 + comp * 8], (double)m / M, c);
+          double rx = 0, rv = 0, ra = 0;
+          for (int j = 4; j < 8; ++j) {
+            rx += std::fabs(c[j]) * std::pow(e, j);
+            rv += j * std::fabs(c[j]) * std::pow(e, j - 1) / h;
+            ra += j * (j - 1) * std::fabs(c[j]) * std::pow(e, j - 2) / (h * h);
+          }
+          ok = rx < 1e-10 && rv < 1e-10 && ra < 1e-8;
+        }
+    if (ok) return M;
+  }
+  return 0;
+}
+
 static int stencil(const double *t, int n, double tq) {
   int il = (int)((tq - t[0]) / (t[1] - t[0]));  // orbit.cu:229 (truncation)
   il = il > 1 ? il : 1;
@@ -409,14 +451,34 @@ int s1ifg_geom_create(s1ifg_ctx *ctx, const s1ifg_orbit *orbit, const s1ifg_burs
     recs[(size_t)(k - 1) * 32] = orbit->t[k];
     for (int m = 0; m < 24; ++m) recs[(size_t)(k - 1) * 32 + 1 + m] = coef[(size_t)(k - 1) * 24 + m];
   }
+  // orbit node table: M + 1 nodes per stencil interval
+  const int M = orbit_node_count(coef, n - 3, h);
+  const size_t nnode = M ? (size_t)(n - 3) * (M + 1) * kNodeRec : 0;
+  std::vector<double> nodes(nnode, 0.0);
+  for (int k = 1; M && k <= n - 3; ++k)
+    for (int m = 0; m <= M; ++m) {
+      double *r = &nodes[((size_t)(k - 1) * (M + 1) + m) * kNodeRec];
+      r[0] = orbit->t[k] + m * (h / M);
+      const double s0 = (r[0] - orbit->t[k]) / h;
+      for (int comp = 0; comp < 3; ++comp) {
+        double c[8];
+        poly_shift(&coef[(size_t)(k - 1) * 24 + comp * 8], s0, c);
+        r[1 + comp] = c[0];
+        r[4 + comp] = c[1] / h;
+        r[7 + comp] = 2 * c[2] / (h * h);
+        r[10 + comp] = 6 * c[3] / (h * h * h);
+      }
+    }
   gd.inv_dmrg = 1.0 / gd.dmrg;
   gd.k4pi_wvl = 4.0 * kPi / gd.wvl;
   gd.inv_dtaz = b->prf;
   gd.inv_samples = 1.0 / N;
   gd.inv_lines = 1.0 / L;
-  // buffer: orbit records | kt | etaref | GeomDev copy (256-byte aligned)
+  // buffer: orbit records | kt | etaref | orbit nodes | GeomDev copy
+  // (256-byte aligned)
   const size_t ndev = (sizeof(GeomDev) + sizeof(double) - 1) / sizeof(double);
-  const size_t off_dev = ((nrec + 2 * (size_t)N) + 31) & ~(size_t)31;
+  const size_t off_node = ((nrec + 2 * (size_t)N) + 31) & ~(size_t)31;
+  const size_t off_dev = (off_node + nnode + 31) & ~(size_t)31;
   size_t nbuf = off_dev + ndev;
   double *d = nullptr;
   S1_HIP(hipMalloc(&d, nbuf * sizeof(double)));
@@ -425,6 +487,8 @@ int s1ifg_geom_create(s1ifg_ctx *ctx, const s1ifg_orbit *orbit, const s1ifg_burs
   gd.orb.h = h;
   gd.orb.inv_h = 1.0 / h;
   gd.orb.n = n;
+  gd.orb.node_m = M;
+  gd.orb.node = M ? d + off_node : nullptr;
   gd.kt = d + nrec;
   gd.etaref = d + nrec + N;
   hipError_t e = hipMemcpy(d, recs.data(), nrec * sizeof(double), hipMemcpyHostToDevice);
@@ -432,6 +496,8 @@ int s1ifg_geom_create(s1ifg_ctx *ctx, const s1ifg_orbit *orbit, const s1ifg_burs
     e = hipMemcpy(d + nrec, kt.data(), N * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(d + nrec + N, etaref.data(), N * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnode)
+    e = hipMemcpy(d + off_node, nodes.data(), nnode * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(d + off_dev, &gd, sizeof(GeomDev), hipMemcpyHostToDevice);
   if (e != hipSuccess) {

@@ -41,7 +41,13 @@ struct OrbitDev {
   double h;      // t_1 - t_0
   double inv_h;  // 1 / h
   int n;         // state vectors
+  int node_m;    // node intervals per stencil interval (0: no node table)
+  const double *node;  // node records, (node_m + 1) per stencil index
 };
+
+// Orbit node record: [tau, x[3], v[3], a[3], j[3], pad[3]] (position and
+// its first three time derivatives at time tau).
+constexpr int kNodeRec = 16;
 
 // Everything a kernel needs about one burst, passed by value.
 struct GeomDev {

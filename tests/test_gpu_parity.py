@@ -105,6 +105,31 @@ def test_geo2rdr_offsets(ctx, name):
     assert np.max(np.abs(_np(az) - (oc["geo_s"]["azoff"] + daz))) < 2e-6
 
 
+def test_geo2rdr_curved_orbit(ctx):
+    """An orbit bent by quadratic to quartic terms: its interval polynomials
+    have higher-order terms, so the Newton iteration's orbit node table
+    holds more nodes per interval.  Offsets equal the oracle's (Hermite
+    evaluation at every step) as on the straight orbits."""
+    from sentinel_gpu_processor_amd import _native as N
+    from sentinel_gpu_processor_amd import kernels as K
+    sc = scene("small")
+    t, pos, vel = sc["orbit_ref"]
+    e = t - t.mean()
+    A = np.array([3.5, -2.0, -2.5])     # m/s^2 / 2
+    B = np.array([2e-3, 1e-3, -3e-3])   # m/s^3 / 6
+    C = np.array([1e-4, -2e-4, 1e-4])   # m/s^4 / 24
+    p2 = pos + A * e[:, None] ** 2 + B * e[:, None] ** 3 + C * e[:, None] ** 4
+    v2 = vel + 2 * A * e[:, None] + 3 * B * e[:, None] ** 2 + 4 * C * e[:, None] ** 3
+    orb = (t, p2, v2)
+    g = N.Geometry(ctx, orb, sc["burst_ref"])
+    dem = sc["dem"]
+    rg, az, rng = K.geo2rdr(ctx, g, sc["grid"], _dev(dem["data"]), dem)
+    o = G.geo2rdr_grid(orb, sc["burst_ref"], sc["grid"], G.target_heights(sc["grid"], dem))
+    assert np.max(np.abs(_np(rg) - o["rgoff"])) < 2e-6
+    assert np.max(np.abs(_np(az) - o["azoff"])) < 2e-6
+    assert np.max(np.abs(_np(rng) - o["rng"])) < 5e-6
+
+
 @pytest.mark.parametrize("interp", ["bilinear", "sinc"])
 @pytest.mark.parametrize("name", ["small", "small_misreg"])
 def test_geocode_matches_oracle(ctx, interp, name):
