@@ -30,9 +30,9 @@ constexpr int kPairWin = 4864;       // staged samples of burst 0 (38 KB; 4 bloc
 constexpr int kPairMaxPix = 4;       // target pixels per look cell on this path
 
 struct PairLds {
-  float2 win[1][kPairWin];
-  int box[2][4];  // per burst: min intr, min inta, max intr, max inta
-  int out[2];     // per burst: every pixel of the tile outside the burst
+  float2 win[kPairWin];  // burst 0's tap window
+  int box[4];            // burst 0: min intr, min inta, max intr, max inta
+  int out[2];            // per burst: every pixel of the tile outside the burst
 };
 
 __device__ __forceinline__ int wave_min_i(int v) {
@@ -58,12 +58,7 @@ __device__ __forceinline__ void pair_tile_sinc(
   const int rm = tile_y * kPairTile + (tid / kPairTile);
   const bool active = cm < nc_ml && rm < nr_ml;
   const int np = rl * cl;
-  int ir[2][kPairMaxPix], ia[2][kPairMaxPix];
-  float fr[2][kPairMaxPix], fa[2][kPairMaxPix], sn[2][kPairMaxPix], cs[2][kPairMaxPix];
-  unsigned vmask = 0;
-  int lo_r[2] = {INT_MAX, INT_MAX}, lo_a[2] = {INT_MAX, INT_MAX};
-  int hi_r[2] = {INT_MIN, INT_MIN}, hi_a[2] = {INT_MIN, INT_MIN};
-  if (tid < 8) L.box[tid >> 2][tid & 3] = (tid & 3) < 2 ? INT_MAX : INT_MIN;
+  if (tid < 4) L.box[tid] = tid < 2 ? INT_MAX : INT_MIN;
   // wavefront b < 2: is the whole tile outside burst b?
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (wave < 2) {
@@ -87,97 +82,106 @@ __device__ __forceinline__ void pair_tile_sinc(
     }
     return;
   }
-  // geometry of the cell's pixels (rows outer, columns inner)
+  // Burst 0: geometry of the cell's pixels (rows outer, columns inner),
+  // their tap window staged in LDS, interpolation.  The pixel heights are
+  // kept for burst 1.
+  double hp[kPairMaxPix];
+  float2 v0[kPairMaxPix];
+  {
+    int ir[kPairMaxPix], ia[kPairMaxPix];
+    float fr[kPairMaxPix], fa[kPairMaxPix], sn[kPairMaxPix], cs[kPairMaxPix];
+    unsigned vmask = 0;
+    int lo_r = INT_MAX, lo_a = INT_MAX, hi_r = INT_MIN, hi_a = INT_MIN;
 #pragma unroll
-  for (int p = 0; p < kPairMaxPix; ++p) {
-    ir[0][p] = ir[1][p] = ia[0][p] = ia[1][p] = 0;
-    fr[0][p] = fr[1][p] = fa[0][p] = fa[1][p] = 0.f;
-    sn[0][p] = sn[1][p] = cs[0][p] = cs[1][p] = 0.f;
-    if (p < np && active) {
-      const int i = p / cl, j = p - i * cl;
-      double X[3];
-      pixel_xyz(gr, dem, rows, cols, dh, rm * rl + i, cm * cl + j, X);
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
+    for (int p = 0; p < kPairMaxPix; ++p) {
+      ir[p] = ia[p] = 0;
+      fr[p] = fa[p] = sn[p] = cs[p] = 0.f;
+      hp[p] = 0.0;
+      if (p < np && active) {
+        const int i = p / cl, j = p - i * cl;
+        const int r = rm * rl + i, c = cm * cl + j;
+        const RowTab rt = rows[r];
+        const ColTab ct = cols[c];
+        hp[p] = dem_height(dem, rt.fy, ct.fx);
+        if (dh) hp[p] += (double)dh[(size_t)r * gr.ncol + c];
+        double X[3];
+        target_xyz(rt, ct, hp[p], X);
         double rg, az, rng;
-        float s, c;
-        if (!(b ? out1 : out0) &&
-            geocode_geom<S1IFG_INTERP_SINC>(b ? G1 : G0, b ? op1 : none, X, rg, az, rng, s, c)) {
+        float s, cc;
+        if (!out0 && geocode_geom<S1IFG_INTERP_SINC>(G0, none, X, rg, az, rng, s, cc)) {
           const int r0 = (int)floor(rg), a0 = (int)floor(az);
-          ir[b][p] = r0;
-          ia[b][p] = a0;
-          fr[b][p] = (float)(rg - r0);
-          fa[b][p] = (float)(az - a0);
-          sn[b][p] = s;
-          cs[b][p] = c;
-          vmask |= 1u << (2 * p + b);
-          lo_r[b] = min(lo_r[b], r0);
-          hi_r[b] = max(hi_r[b], r0);
-          lo_a[b] = min(lo_a[b], a0);
-          hi_a[b] = max(hi_a[b], a0);
+          ir[p] = r0;
+          ia[p] = a0;
+          fr[p] = (float)(rg - r0);
+          fa[p] = (float)(az - a0);
+          sn[p] = s;
+          cs[p] = cc;
+          vmask |= 1u << p;
+          lo_r = min(lo_r, r0);
+          hi_r = max(hi_r, r0);
+          lo_a = min(lo_a, a0);
+          hi_a = max(hi_a, a0);
         }
       }
     }
-  }
-#pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    const int v0 = wave_min_i(lo_r[b]), v1 = wave_min_i(lo_a[b]);
-    const int v2 = wave_max_i(hi_r[b]), v3 = wave_max_i(hi_a[b]);
-    if ((tid & 63) == 0) {
-      atomicMin(&L.box[b][0], v0);
-      atomicMin(&L.box[b][1], v1);
-      atomicMax(&L.box[b][2], v2);
-      atomicMax(&L.box[b][3], v3);
+    {
+      const int w0 = wave_min_i(lo_r), w1 = wave_min_i(lo_a);
+      const int w2 = wave_max_i(hi_r), w3 = wave_max_i(hi_a);
+      if ((tid & 63) == 0) {
+        atomicMin(&L.box[0], w0);
+        atomicMin(&L.box[1], w1);
+        atomicMax(&L.box[2], w2);
+        atomicMax(&L.box[3], w3);
+      }
     }
-  }
-  __syncthreads();
-  // stage the tap windows (columns min-3 .. max+4, rows likewise)
-  int wc[2], wr0[2], wa0[2];
-  bool staged[2];
-#pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    const int mr = L.box[b][0], ma = L.box[b][1], xr = L.box[b][2], xa = L.box[b][3];
+    __syncthreads();
+    // stage the tap window (columns min-3 .. max+4, rows likewise)
+    const int mr = L.box[0], ma = L.box[1], xr = L.box[2], xa = L.box[3];
     const bool any = mr <= xr;
     const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
-    // burst 0 from LDS, burst 1 by gathers through the vector L1: the
-    // two memory paths run side by side
-    staged[b] = b == 0 && any && W * H <= kPairWin;
-    wc[b] = (int)W;
-    wr0[b] = mr - 3;
-    wa0[b] = ma - 3;
-    if (staged[b]) {
-      const GeomDev &g = b ? G1 : G0;
-      const float2 *z = b ? z1 : z0;
+    const bool staged = any && W * H <= kPairWin;
+    const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
+    if (staged)
       for (int row = tid >> 6; row < H; row += 4)
         for (int col = tid & 63; col < W; col += 64)
-          L.win[0][row * W + col] = z[(size_t)(wa0[b] + row) * g.samples + wr0[b] + col];
+          L.win[row * W + col] = z0[(size_t)(wa0 + row) * G0.samples + wr0 + col];
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < kPairMaxPix; ++p) {
+      v0[p] = make_float2(0.f, 0.f);
+      if (p < np && (vmask & (1u << p))) {
+        if (staged)
+          v0[p] = sinc_interp(&L.win[(ia[p] - 3 - wa0) * wc + (ir[p] - 3 - wr0)], wc, fr[p],
+                              fa[p], sn[p], cs[p]);
+        else
+          v0[p] = sinc_interp(z0 + (size_t)(ia[p] - 3) * G0.samples + (ir[p] - 3), G0.samples,
+                              fr[p], fa[p], sn[p], cs[p]);
+      }
     }
   }
-  __syncthreads();
-  // interpolation and look means, in the order of pair_ml_pixel
+  // Burst 1, pixel by pixel: geometry, interpolation from global memory
+  // (vector L1: the two memory paths share the taps), and the look means
+  // in the order of pair_ml_pixel.
   float ax = 0.f, ay = 0.f, apr = 0.f, aps = 0.f;
   float sx = 0.f, sy = 0.f, spr = 0.f, sps = 0.f;
   const float inv_cl = 1.0f / cl, inv_rl = 1.0f / rl;
 #pragma unroll
   for (int p = 0; p < kPairMaxPix; ++p) {
     if (p < np) {
-      float2 v[2];
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        v[b] = make_float2(0.f, 0.f);
-        if (vmask & (1u << (2 * p + b))) {
-          if (staged[b]) {
-            v[b] = sinc_interp(&L.win[0][(ia[b][p] - 3 - wa0[b]) * wc[b] + (ir[b][p] - 3 - wr0[b])],
-                               wc[b], fr[b][p], fa[b][p], sn[b][p], cs[b][p]);
-          } else {
-            const GeomDev &g = b ? G1 : G0;
-            const float2 *z = b ? z1 : z0;
-            v[b] = sinc_interp(z + (size_t)(ia[b][p] - 3) * g.samples + (ir[b][p] - 3), g.samples,
-                               fr[b][p], fa[b][p], sn[b][p], cs[b][p]);
-          }
+      float2 c = make_float2(0.f, 0.f);
+      if (active && !out1) {
+        const int i = p / cl, j = p - i * cl;
+        double X[3];
+        target_xyz(rows[rm * rl + i], cols[cm * cl + j], hp[p], X);
+        double rg, az, rng;
+        float s, cc;
+        if (geocode_geom<S1IFG_INTERP_SINC>(G1, op1, X, rg, az, rng, s, cc)) {
+          const int r0 = (int)floor(rg), a0 = (int)floor(az);
+          c = sinc_interp(z1 + (size_t)(a0 - 3) * G1.samples + (r0 - 3), G1.samples,
+                          (float)(rg - r0), (float)(az - a0), s, cc);
         }
       }
-      const float2 a = v[0], c = v[1];
+      const float2 a = v0[p];
       sx += a.x * c.x + a.y * c.y;
       sy += a.x * c.y - a.y * c.x;
       spr += a.x * a.x + a.y * a.y;
