@@ -667,7 +667,8 @@ def main():
     roof = dict(kernel=dom, bound="hbm", achieved=k["achieved_gbs"], peak=HBM_PEAK / 1e9,
                 unit="GB/s", frac=k["hbm_frac"], traffic=k["traffic"],
                 alg_bytes_per_launch=sb[dom], valu_issue_busy=k["valu_issue_busy"],
-                limiter=("VALU issue: per-pixel FP64 geometry and FP32 8x8 sinc taps"
+                limiter=("latency and issue: per-pixel FP64 geometry chains, and 2 x 64 sinc "
+                         "taps per pixel through LDS (burst 0) and vector-L1 gathers (burst 1)"
                          if dom in ("pair", "geocode") else None),
                 kernels=kernels)
     if not args.no_calibrate_copy:
