@@ -1,22 +1,22 @@
-// Fused burst pair with sinc interpolation, burst windows staged in LDS
-// (shared by the per-pair and batched kernels).
+// Fused burst pair with sinc interpolation, burst 0's tap windows staged
+// in LDS (shared by the per-pair and batched kernels).
 //
 // A 256-thread block owns a tile of 16 x 16 look cells.  Two wavefronts
 // first test whether the whole tile lies outside either burst
 // (tile_outside: about half of a burst's target grid, the corners of its
 // latitude / longitude box, does); a burst that the tile misses is not
 // geocoded, and a tile that misses both writes zeros.  Each thread owns
-// one cell (rl * cl <= kPairMaxPix target pixels): it first computes the
-// FP64 geometry and phase of its pixels in both bursts and keeps the
+// one cell (rl * cl <= kPairMaxPix target pixels).  Burst 0: the thread
+// computes the FP64 geometry and phase of its pixels and keeps the
 // integer / fractional sampling positions and sin/cos of the phase in
-// registers.  The block then reduces the bounding box of the 8 x 8 tap
-// windows of each burst, loads that window of the deramped burst into LDS
-// with row-contiguous (coalesced) loads, and every pixel is interpolated
-// from LDS.  A window larger than kPairWin samples (steep terrain spreads
-// the range positions) is read from global memory instead.  Both ways run
-// the same arithmetic in the same order as pair_ml_pixel, so the products
-// are bit-identical to it: the staging only moves data (each burst sample
-// is fetched about 3 times per tile instead of 64 tap gathers per pixel).
+// registers; the block reduces the bounding box of the 8 x 8 tap windows,
+// loads that window of the deramped burst into LDS with row-contiguous
+// (coalesced) loads, and every pixel is interpolated from LDS (a window
+// larger than kPairWin samples is read from global memory instead).
+// Burst 1: pixel by pixel, geometry and interpolation straight from global
+// memory, so its taps go through the vector L1 while burst 0's go through
+// LDS.  All ways run the same arithmetic in the same order as
+// pair_ml_pixel, so the products are bit-identical to it.
 #pragma once
 
 #include <climits>
