@@ -156,29 +156,32 @@ __global__ void kb_reset(unsigned long long *ctr, int n) {
 }
 
 // Sinc pair kernel with LDS-staged windows over 16 x 16-cell tiles of every
-// pair (rl * cl <= kPairMaxPix).  Persistent blocks take tiles from a work
-// queue counter, so the tiles in flight on the chip stay together in the
-// bursts (cache reuse) whatever the grid size; the next tile is fetched
-// while the current one runs.  A grid with a block per tile runs one tile
-// per block in block order without the queue.
+// pair (rl * cl <= kPairMaxPix), one block per tile in tile order.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_pair_tiles(
+    const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, int rl, int cl) {
+  __shared__ PairLds L;
+  const long t = blockIdx.x;
+  if (t >= off[n]) return;
+  const int p = find_item(off, n, t);
+  const PairDev &d = *(const PairDev *)&P[p];
+  const long lt = t - off[p];
+  const int tiles_x = (d.nc_ml + kPairTile - 1) / kPairTile;
+  const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
+  pair_tile_sinc(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
+                 d.sec_d, rl, cl, ty, tx, d.nr_ml, d.nc_ml, d.ifg, d.pr, d.ps, L);
+}
+
+// The same with persistent blocks that take tiles from a work-queue
+// counter, so the tiles in flight on the chip stay together in the bursts
+// (cache reuse) whatever the grid size; the next tile is fetched while the
+// current one runs.  (A separate kernel: one inlined copy of the tile code
+// per kernel keeps either within the register budget.)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_pair_tiles_q(
     const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, int rl, int cl,
     unsigned long long *ctr) {
   __shared__ PairLds L;
   __shared__ long slot[2];
   const long total = off[n];
-  if (gridDim.x >= total) {
-    // one tile per block, no queue
-    const long t = blockIdx.x;
-    const int p = find_item(off, n, t);
-    const PairDev &d = *(const PairDev *)&P[p];
-    const long lt = t - off[p];
-    const int tiles_x = (d.nc_ml + kPairTile - 1) / kPairTile;
-    const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
-    pair_tile_sinc(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
-                   d.sec_d, rl, cl, ty, tx, d.nr_ml, d.nc_ml, d.ifg, d.pr, d.ps, L);
-    return;
-  }
   if (threadIdx.x == 0) slot[0] = (long)atomicAdd(ctr, 1ull);
   __syncthreads();
   int cur = 0, p = -1;
@@ -375,8 +378,11 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
   if (long t = total(kStPair)) {
     dim3 blk(o.pair_block_x, 256 / o.pair_block_x);
     dim3 grd(grid_of(t, 1, ncu, o.pair_blocks_per_cu));
-    if (use_pair_tiles(o))
+    if (use_pair_tiles(o) && (long)grd.x >= t)
       hipLaunchKernelGGL(kb_pair_tiles, grd, dim3(256), 0, s, pairs, off[kStPair], n, o.rl,
+                         o.cl);
+    else if (use_pair_tiles(o))
+      hipLaunchKernelGGL(kb_pair_tiles_q, grd, dim3(256), 0, s, pairs, off[kStPair], n, o.rl,
                          o.cl, b->d_ctr + kStPair);
     else if (o.interp == S1IFG_INTERP_BILINEAR)
       hipLaunchKernelGGL(kb_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, s, pairs, off[kStPair],
