@@ -306,11 +306,13 @@ __global__ __launch_bounds__(kAdjBlock) void k_pair_bwd(
 // per burst (worst seen ~1,900) where 16 x 16 needed ~1,900 (~3,500).
 constexpr int kBwdRows = 8, kBwdCols = 32, kBwdThreads = kBwdRows * kBwdCols;
 constexpr int kBwdWin = 2048;
+// Skip regions: 2 x 2 tiles (16 x 64 target pixels) share one skip flag per
+// burst, tested by k_bwd_flags before the tiles run.
+constexpr int kBwdRegion = 2;
 
 struct BwdLds {
   float2 win[2][kBwdWin];
   int box[2][4];
-  int out[2];  // per burst: every pixel of the tile outside the burst
   double red[kBwdThreads / 64][6];
 };
 
@@ -323,12 +325,34 @@ __device__ __forceinline__ int wmax_i(int v) {
   return v;
 }
 
+// Skip flags of the backward's regions (2 x 2 tiles): wavefront w of the
+// grid tests region w / 2 (row-major, rx per row) against burst w % 2
+// (tile_outside), over the region's pixels inside the look cells.
+__global__ __launch_bounds__(256) void k_bwd_flags(
+    const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr,
+    DemDev dem, const RowTab *rows, const ColTab *cols, const float *dh, int rl, int cl,
+    int nr_ml, int nc_ml, int rx, long nflags, unsigned char *__restrict__ flags) {
+  const OffPoly none = {{0, 0, 0, 0, 0, 0}, 0};
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nflags) return;
+  const long t = w >> 1;
+  const int b = (int)(w & 1);
+  const int ry = (int)(t / rx), cx = (int)(t - (long)ry * rx);
+  const int r0 = ry * kBwdRegion * kBwdRows, c0 = cx * kBwdRegion * kBwdCols;
+  const int r1 = min(r0 + kBwdRegion * kBwdRows, nr_ml * rl) - 1;
+  const int c1 = min(c0 + kBwdRegion * kBwdCols, nc_ml * cl) - 1;
+  const bool o = tile_outside<S1IFG_INTERP_SINC>(b ? *gps : *gpr, b ? ops : none, dem, rows, cols,
+                                                 dh, gr.ncol, r0, r1, c0, c1);
+  if ((threadIdx.x & 63) == 0) flags[w] = o;
+}
+
 __global__ __launch_bounds__(kBwdThreads) void k_pair_bwd_tiles(
     const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr,
     DemDev dem, const RowTab *rows, const ColTab *cols, const float *dh,
     const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl, int cl, int nr_ml,
     int nc_ml, const float2 *__restrict__ g_ifg, const float *__restrict__ g_pr,
-    const float *__restrict__ g_ps, double *__restrict__ partial, float *__restrict__ g_dh) {
+    const float *__restrict__ g_ps, double *__restrict__ partial, float *__restrict__ g_dh,
+    const unsigned char *__restrict__ flags) {
   __shared__ BwdLds L;
   const GeomDev *G[2] = {gpr, gps};
   const float2 *Z[2] = {zr, zs};
@@ -338,17 +362,10 @@ __global__ __launch_bounds__(kBwdThreads) void k_pair_bwd_tiles(
   const int c = blockIdx.x * kBwdCols + (tid & (kBwdCols - 1));
   const bool active = r < nr_ml * rl && c < nc_ml * cl;
   if (tid < 8) L.box[tid >> 2][tid & 3] = (tid & 3) < 2 ? INT_MAX : INT_MIN;
-  // wavefront k < 2: is the whole tile outside burst k?  (tile_outside)
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  if (wave < 2) {
-    const int r0 = blockIdx.y * kBwdRows, c0 = blockIdx.x * kBwdCols;
-    const int r1 = min(r0 + kBwdRows, nr_ml * rl) - 1, c1 = min(c0 + kBwdCols, nc_ml * cl) - 1;
-    const bool o = tile_outside<S1IFG_INTERP_SINC>(wave ? *gps : *gpr, wave ? ops : none, dem,
-                                                   rows, cols, dh, gr.ncol, r0, r1, c0, c1);
-    if ((tid & 63) == 0) L.out[wave] = o;
-  }
-  __syncthreads();  // box initialised, L.out set
-  const bool out[2] = {L.out[0] != 0, L.out[1] != 0};
+  // the skip flags of the tile's region (k_bwd_flags)
+  const long reg = (long)(blockIdx.y / kBwdRegion) * ((gridDim.x + kBwdRegion - 1) / kBwdRegion) +
+                   blockIdx.x / kBwdRegion;
+  const bool out[2] = {flags[2 * reg] != 0, flags[2 * reg + 1] != 0};
   if (out[0] && out[1]) {
     // no pixel of the tile in either burst: zero gradients
     if (active && g_dh) g_dh[(size_t)r * gr.ncol + c] = 0.f;
@@ -382,6 +399,7 @@ __global__ __launch_bounds__(kBwdThreads) void k_pair_bwd_tiles(
       }
     }
   }
+  __syncthreads();  // box initialised
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int v0 = wmin_i(lo_r[k]), v1 = wmin_i(lo_a[k]);
@@ -652,10 +670,14 @@ int s1ifg_pair_backward(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geo
   RowTab *rows;
   ColTab *cols;
   void *part = nullptr;
-  // per-block partials, then kReduceBlocks first-level sums
-  if ((rc = grid_tables(ctx, gr, dd, &rows, &cols,
-                        sizeof(double) * 6 * ((size_t)blocks + kReduceBlocks), &part)))
+  // per-block partials, then kReduceBlocks first-level sums, then the skip
+  // flags of the tile regions
+  const int rx = (int)((tgrid.x + kBwdRegion - 1) / kBwdRegion);
+  const long nflags = tiles ? 2L * rx * ((tgrid.y + kBwdRegion - 1) / kBwdRegion) : 0;
+  const size_t part_bytes = sizeof(double) * 6 * ((size_t)blocks + kReduceBlocks);
+  if ((rc = grid_tables(ctx, gr, dd, &rows, &cols, part_bytes + (size_t)nflags, &part)))
     return rc;
+  unsigned char *flags = (unsigned char *)part + part_bytes;
   if (g_dh) {
     long n = (long)gr.nrow * gr.ncol;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -667,11 +689,17 @@ int s1ifg_pair_backward(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geo
                        ctx->stream, g_ref->d_dev, g_sec->d_dev, op, gr, dd, rows, cols, dh,
                        (const float2 *)ref_deramped, (const float2 *)sec_deramped, rl, cl, nr, nc,
                        (const float2 *)g_ifg, g_pow_ref, g_pow_sec, (double *)part, g_dh);
-  else
+  else {
+    hipLaunchKernelGGL(k_bwd_flags, dim3((unsigned)((nflags + 3) / 4)), dim3(256), 0,
+                       ctx->stream, g_ref->d_dev, g_sec->d_dev, op, gr, dd, rows, cols, dh, rl,
+                       cl, nr, nc, rx, nflags, flags);
+    S1_LAUNCH_CHECK("k_bwd_flags");
     hipLaunchKernelGGL(k_pair_bwd_tiles, tgrid, dim3(kBwdThreads), 0, ctx->stream,
                        g_ref->d_dev, g_sec->d_dev, op, gr, dd, rows, cols, dh,
                        (const float2 *)ref_deramped, (const float2 *)sec_deramped, rl, cl, nr, nc,
-                       (const float2 *)g_ifg, g_pow_ref, g_pow_sec, (double *)part, g_dh);
+                       (const float2 *)g_ifg, g_pow_ref, g_pow_sec, (double *)part, g_dh,
+                       (const unsigned char *)flags);
+  }
   S1_LAUNCH_CHECK("k_pair_bwd");
   if (g_offpoly) {
     double *lvl = (double *)part + 6 * (size_t)blocks;
