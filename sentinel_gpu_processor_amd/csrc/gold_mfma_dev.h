@@ -69,19 +69,27 @@ __device__ __forceinline__ half8 neg8(half8 v) {
   return __builtin_bit_cast(half8, __builtin_bit_cast(u32x4, v) ^ 0x80008000u);
 }
 
-// hi + lo split of v (|v| < 2^16).  lo is one mixed-precision FMA
-// v * 1 - hi per value (the 1 is hidden from constant folding so that the
-// FMA is not rewritten as a subtraction with separate conversions).
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+// hi + lo split of v (|v| < 2^16).  hi rounds toward zero, two values per
+// packing conversion; lo = v - hi is exact in FP32 (|lo| < ulp(hi)) and is
+// rounded to fp16 by one mixed-precision FMA v * 1 - hi per value that
+// reads hi from its packed half (the 1 is hidden from constant folding so
+// that the FMA is not rewritten as a subtraction with separate
+// conversions).
 __device__ __forceinline__ void split16(const float v[16], half8 hi[2], half8 lo[2]) {
   float one = 1.0f;
   asm("" : "+v"(one));
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const _Float16 h = (_Float16)v[8 * c + j];
-      hi[c][j] = h;
-      lo[c][j] = (_Float16)fmaf(v[8 * c + j], one, -(float)h);
+    for (int j = 0; j < 8; j += 2) {
+      const half2v h =
+          __builtin_bit_cast(half2v, __builtin_amdgcn_cvt_pkrtz(v[8 * c + j], v[8 * c + j + 1]));
+      hi[c][j] = h[0];
+      hi[c][j + 1] = h[1];
+      lo[c][j] = (_Float16)fmaf(v[8 * c + j], one, -(float)h[0]);
+      lo[c][j + 1] = (_Float16)fmaf(v[8 * c + j + 1], one, -(float)h[1]);
     }
 }
 
