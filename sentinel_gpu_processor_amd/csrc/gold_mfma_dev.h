@@ -202,16 +202,38 @@ __device__ __forceinline__ void cgemm_v(const float xr[16], const float xi[16], 
   cgemm<CONJ>(arh, arl, aih, ail, b, yr, yi);
 }
 
+// Wavefront reductions with the result in every lane, on the VALU:
+// DPP within rows of 16 lanes (quad permutations, then the half-row and row
+// mirrors, which pair quads and half rows once their lanes agree), then
+// the gfx950 lane swaps between rows 0/1 and 2/3 (permlane16_swap) and
+// between the two halves (permlane32_swap).  Called by full wavefronts.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float x) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF,
+                                                        false));
+}
+
+template <typename Op>
+__device__ __forceinline__ float wave_reduce(float v, Op op) {
+  v = op(v, dpp_mov<0xB1>(v));   // quad_perm [1, 0, 3, 2]
+  v = op(v, dpp_mov<0x4E>(v));   // quad_perm [2, 3, 0, 1]
+  v = op(v, dpp_mov<0x141>(v));  // row_half_mirror
+  v = op(v, dpp_mov<0x140>(v));  // row_mirror
+  const unsigned u = __builtin_bit_cast(unsigned, v);
+  const auto r16 = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+  v = op(__builtin_bit_cast(float, (unsigned)r16[0]), __builtin_bit_cast(float, (unsigned)r16[1]));
+  const unsigned w = __builtin_bit_cast(unsigned, v);
+  const auto r32 = __builtin_amdgcn_permlane32_swap(w, w, false, false);
+  return op(__builtin_bit_cast(float, (unsigned)r32[0]), __builtin_bit_cast(float, (unsigned)r32[1]));
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  return wave_reduce(v, [](float a, float b) { return a + b; });
 }
 
 __device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-  return v;
+  return wave_reduce(v, [](float a, float b) { return fmaxf(a, b); });
 }
 
 // Patch p of an image: row n, columns col(r) (goldstein.cu:298-323 gather).
