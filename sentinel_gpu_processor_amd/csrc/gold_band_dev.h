@@ -36,6 +36,7 @@ constexpr int kBandWaves = 4;                        // wavefronts per block
 
 struct BandLds {
   float2 buf[kBandWaves][kBandH * kBandLd];
+  mf::SmoothLds u[kBandWaves];
 };
 
 __device__ __forceinline__ int gold_origin(int j, int n) { return min(j * (kGoldN / 4), n - kGoldN); }
@@ -61,10 +62,10 @@ __device__ __forceinline__ void wave_sync_lds() {
 // One (group, chunk) item on one wavefront (no block barriers).  buf: the
 // wave's LDS buffer (kBandH x kBandLd); partial: this image's partial buffer
 // [ngroup][nchunk][kBandH][kBandCols].
-__device__ __forceinline__ void band_item(const mf::Consts &K, const float2 *__restrict__ img,
-                                          int nrow, int ncol, int pi_n, int pj_n, int g, int c,
-                                          float alpha, float2 *__restrict__ partial,
-                                          float2 *buf) {
+__device__ __forceinline__ void band_item(const mf::Consts &K, const mf::SmoothLds &U,
+                                          const float2 *__restrict__ img, int nrow, int ncol,
+                                          int pi_n, int pj_n, int g, int c, float alpha,
+                                          float2 *__restrict__ partial, float2 *buf) {
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5, n = lane & 31;
   const int nchunk = band_nchunk(pj_n);
@@ -86,9 +87,9 @@ __device__ __forceinline__ void band_item(const mf::Consts &K, const float2 *__r
       mf::load_patch(img, nrow, ncol, pj_n, (long)p * pj_n + j, h, n, x);
       float2 out[16];
       if (alpha == 0.5f)
-        mf::filter_patch_regs<true>(K, x, h, n, alpha, out);
+        mf::filter_patch_regs<true>(K, U, x, h, n, alpha, out);
       else
-        mf::filter_patch_regs<false>(K, x, h, n, alpha, out);
+        mf::filter_patch_regs<false>(K, U, x, h, n, alpha, out);
       // add the patch (within a patch each lane owns distinct entries)
       float2 *row = buf + (gold_origin(p, nrow) - oy0 + n) * kBandLd;
 #pragma unroll

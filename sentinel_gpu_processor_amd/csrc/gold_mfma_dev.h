@@ -64,11 +64,6 @@ __device__ __forceinline__ int kappa(int c, int h, int j) {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// -v for 8 packed fp16 (sign bits flipped, one XOR per dword).
-__device__ __forceinline__ half8 neg8(half8 v) {
-  return __builtin_bit_cast(half8, __builtin_bit_cast(u32x4, v) ^ 0x80008000u);
-}
-
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 
 // hi + lo split of v (|v| < 2^16).  hi rounds toward zero, two values per
@@ -100,22 +95,29 @@ __device__ __forceinline__ void to_half16(const float v[16], float s, half8 hi[2
     for (int j = 0; j < 8; ++j) hi[c][j] = (_Float16)(v[8 * c + j] * s);
 }
 
-struct CplxOp {  // constant complex B operand, split, per chunk
-  half8 rh[2], rl[2], ih[2], il[2];
+struct CplxOp {  // constant complex B operand, split, per chunk; -Im too
+  half8 rh[2], rl[2], ih[2], il[2], nih[2], nil[2];
 };
 
 struct Consts {
   CplxOp F;
-  half8 u1h[2], u1l[2], u2h[2];
 };
+
+// The smoothing operands, in LDS between uses (they are needed once per
+// patch): per wavefront [6][64] half8, entry [k][lane] with k = u1h[0],
+// u1h[1], u1l[0], u1l[1], u2h[0], u2h[1].
+typedef half8 SmoothLds[6][64];
 
 __device__ __forceinline__ void split1(float x, _Float16 &h, _Float16 &l) {
   h = (_Float16)x;
   l = (_Float16)(x - (float)h);
 }
 
-// Constant B operands of lane (h, n) in kappa order.
-__device__ __forceinline__ void build_consts(int h, int n, Consts &K) {
+// Constant B operands of lane (h, n) in kappa order; the smoothing
+// operands go to U (this wavefront's; visible to it after a wavefront
+// barrier).
+__device__ __forceinline__ void build_consts(int h, int n, Consts &K, SmoothLds &U) {
+  half8 u1h[2], u1l[2], u2h[2];
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -131,19 +133,28 @@ __device__ __forceinline__ void build_consts(int h, int n, Consts &K) {
       split1(-sn, a, b);  // Im F = -sin
       K.F.ih[c][j] = a;
       K.F.il[c][j] = b;
+      K.F.nih[c][j] = -a;
+      K.F.nil[c][j] = -b;
       const int d = ((k - n + 16) & 31) - 16;  // circular offset
       const float b1 = (d >= -3 && d <= 3) ? kU1[d + 3] : 0.f;
       const float b2 = (d >= -3 && d <= 3) ? kU2[d + 3] : 0.f;
       split1(b1, a, b);
-      K.u1h[c][j] = a;
-      K.u1l[c][j] = b;
-      K.u2h[c][j] = (_Float16)b2;
+      u1h[c][j] = a;
+      u1l[c][j] = b;
+      u2h[c][j] = (_Float16)b2;
     }
+  const int lane = 32 * h + n;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    U[c][lane] = u1h[c];
+    U[2 + c][lane] = u1l[c];
+    U[4 + c][lane] = u2h[c];
+  }
 }
 
 // C = A * B (B = F) or A * conj(B) (CONJ), A given by split real/imag
-// parts, B constant.  The minus signs of the complex product go on a
-// negated copy of A's imaginary (B = F) or real (CONJ) part.  One
+// parts, B constant.  The minus signs of the complex product come with the
+// negated copy of B's imaginary part (fp16 negation is exact).  One
 // accumulator for the real and one for the imaginary part, alternating, the
 // hi*lo terms of a K chunk before its hi*hi terms.
 template <bool CONJ>
@@ -155,34 +166,32 @@ __device__ __forceinline__ void cgemm(const half8 arh[2], const half8 arl[2], co
   for (int c = 0; c < 2; ++c) {
     if (!CONJ) {
       // Cr = Ar Fr - Ai Fi, Ci = Ar Fi + Ai Fr
-      const half8 nih = neg8(aih[c]), nil = neg8(ail[c]);
       re = mfma(arl[c], b.rh[c], re);
       im = mfma(arl[c], b.ih[c], im);
       re = mfma(arh[c], b.rl[c], re);
       im = mfma(arh[c], b.il[c], im);
-      re = mfma(nil, b.ih[c], re);
+      re = mfma(ail[c], b.nih[c], re);
       im = mfma(ail[c], b.rh[c], im);
-      re = mfma(nih, b.il[c], re);
+      re = mfma(aih[c], b.nil[c], re);
       im = mfma(aih[c], b.rl[c], im);
       re = mfma(arh[c], b.rh[c], re);
       im = mfma(arh[c], b.ih[c], im);
-      re = mfma(nih, b.ih[c], re);
+      re = mfma(aih[c], b.nih[c], re);
       im = mfma(aih[c], b.rh[c], im);
     } else {
       // Cr = Ar Fr + Ai Fi, Ci = Ai Fr - Ar Fi
-      const half8 nrh = neg8(arh[c]), nrl = neg8(arl[c]);
       re = mfma(arl[c], b.rh[c], re);
       im = mfma(ail[c], b.rh[c], im);
       re = mfma(arh[c], b.rl[c], re);
       im = mfma(aih[c], b.rl[c], im);
       re = mfma(ail[c], b.ih[c], re);
-      im = mfma(nrl, b.ih[c], im);
+      im = mfma(arl[c], b.nih[c], im);
       re = mfma(aih[c], b.il[c], re);
-      im = mfma(nrh, b.il[c], im);
+      im = mfma(arh[c], b.nil[c], im);
       re = mfma(arh[c], b.rh[c], re);
       im = mfma(aih[c], b.rh[c], im);
       re = mfma(aih[c], b.ih[c], re);
-      im = mfma(nrh, b.ih[c], im);
+      im = mfma(arh[c], b.nih[c], im);
     }
   }
 #pragma unroll
@@ -255,8 +264,10 @@ __device__ __forceinline__ void load_patch(const float2 *__restrict__ img, int n
 // SQRT: alpha = 0.5 (the usual Goldstein exponent), the weight is one
 // square root.
 template <bool SQRT>
-__device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 x[16], int h,
-                                                  int n, float alpha, float2 out[16]) {
+__device__ __forceinline__ void filter_patch_regs(const Consts &K, const SmoothLds &U,
+                                                  const float2 x[16], int h, int n, float alpha,
+                                                  float2 out[16]) {
+  const int lane = 32 * h + n;
   float xr[16], xi[16];
   float mx = 0.f;
 #pragma unroll
@@ -281,13 +292,21 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 
   for (int r = 0; r < 16; ++r) m[r] = __builtin_amdgcn_sqrtf(zr[r] * zr[r] + zi[r] * zi[r]);
   half8 mh[2], ml[2];
   split16(m, mh, ml);
+  half8 u1h[2], u1l[2], u2h[2];
+  asm volatile("" ::: "memory");  // read here, not held in registers across patches
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    u1h[c] = U[c][lane];
+    u1l[c] = U[2 + c][lane];
+    u2h[c] = U[4 + c][lane];
+  }
   f32x16 t1 = {}, t2 = {};
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    t1 = mfma(mh[c], K.u1l[c], t1);
-    t2 = mfma(mh[c], K.u2h[c], t2);
-    t1 = mfma(ml[c], K.u1h[c], t1);
-    t1 = mfma(mh[c], K.u1h[c], t1);
+    t1 = mfma(mh[c], u1l[c], t1);
+    t2 = mfma(mh[c], u2h[c], t2);
+    t1 = mfma(ml[c], u1h[c], t1);
+    t1 = mfma(mh[c], u1h[c], t1);
   }
   // S = l1 (B1^T M B1 + (l2/l1) B2^T M B2): the second pass runs on B_r,
   // l2/l1 scales the rank-2 term's data and l1 the sum
@@ -303,10 +322,10 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const float2 
   f32x16 sm = {};
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    sm = mfma(t2h[c], K.u2h[c], sm);
-    sm = mfma(t1h[c], K.u1l[c], sm);
-    sm = mfma(t1l[c], K.u1h[c], sm);
-    sm = mfma(t1h[c], K.u1h[c], sm);
+    sm = mfma(t2h[c], u2h[c], sm);
+    sm = mfma(t1h[c], u1l[c], sm);
+    sm = mfma(t1l[c], u1h[c], sm);
+    sm = mfma(t1h[c], u1h[c], sm);
   }
   float s[16];
   float lsum = 0.f;

@@ -139,13 +139,14 @@ __global__ __launch_bounds__(64 * kBandWaves) __attribute__((amdgpu_waves_per_eu
   long t = (long)blockIdx.x * kBandWaves + wave;
   if (t >= total) return;
   mf::Consts K;
-  mf::build_consts(lane >> 5, lane & 31, K);
+  mf::build_consts(lane >> 5, lane & 31, K, L.u[wave]);
+  wave_sync_lds();  // L.u[wave] written
   for (; t < total; t += (long)gridDim.x * kBandWaves) {
     const int p = find_item(off, n, t);
     const PairDev &d = *(const PairDev *)&P[p];
     const long lt = t - off[p];
     const int nchunk = band_nchunk(d.pj_n);
-    band_item(K, d.ifg, d.nr_ml, d.nc_ml, d.pi_n, d.pj_n, (int)(lt / nchunk), (int)(lt % nchunk),
+    band_item(K, L.u[wave], d.ifg, d.nr_ml, d.nc_ml, d.pi_n, d.pj_n, (int)(lt / nchunk), (int)(lt % nchunk),
               alpha, d.patches, L.buf[wave]);
   }
 }

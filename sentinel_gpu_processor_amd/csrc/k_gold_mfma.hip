@@ -17,9 +17,10 @@ __global__ __launch_bounds__(64 * kBandWaves) __attribute__((amdgpu_waves_per_eu
   long t = (long)blockIdx.x * kBandWaves + wave;
   if (t >= total) return;
   mf::Consts K;
-  mf::build_consts(lane >> 5, lane & 31, K);
+  mf::build_consts(lane >> 5, lane & 31, K, L.u[wave]);
+  wave_sync_lds();  // L.u[wave] written
   for (; t < total; t += (long)gridDim.x * kBandWaves)
-    band_item(K, img, nrow, ncol, pi_n, pj_n, (int)(t / nchunk), (int)(t % nchunk), alpha,
+    band_item(K, L.u[wave], img, nrow, ncol, pi_n, pj_n, (int)(t / nchunk), (int)(t % nchunk), alpha,
               partial, L.buf[wave]);
 }
 
