@@ -86,20 +86,23 @@ __device__ __forceinline__ void band_item(const mf::Consts &K, const mf::SmoothL
       float2 x[16];
       mf::load_patch(img, nrow, ncol, pj_n, (long)p * pj_n + j, h, n, x);
       float2 out[16];
+      float norm;
       if (alpha == 0.5f)
-        mf::filter_patch_regs<true>(K, U, x, h, n, alpha, out);
+        mf::filter_patch_regs<true>(K, U, x, h, n, alpha, out, norm);
       else
-        mf::filter_patch_regs<false>(K, U, x, h, n, alpha, out);
-      // add the patch (within a patch each lane owns distinct entries)
+        mf::filter_patch_regs<false>(K, U, x, h, n, alpha, out, norm);
+      // add the weighted patch (within a patch each lane owns distinct
+      // entries)
       float2 *row = buf + (gold_origin(p, nrow) - oy0 + n) * kBandLd;
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           float2 *e = row + ((ox + 4 * h + 8 * q + t) & (kBandBuf - 1));
+          const float w = mf::patch_weight(h, n, q, t, norm);
           float2 v = *e;
-          v.x += out[4 * q + t].x;
-          v.y += out[4 * q + t].y;
+          v.x = fmaf(out[4 * q + t].x, w, v.x);
+          v.y = fmaf(out[4 * q + t].y, w, v.y);
           *e = v;
         }
       wave_sync_lds();  // the next patch's lanes add to these entries

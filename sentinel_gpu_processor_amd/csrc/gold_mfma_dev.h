@@ -258,15 +258,16 @@ __device__ __forceinline__ void load_patch(const float2 *__restrict__ img, int n
     for (int t = 0; t < 4; ++t) v[4 * q + t] = src[8 * q + t];
 }
 
-// Filter one patch held in x (lane layout) into its weighted result
-// w(row, col) z'/n^2 in the same layout: lane (h, n) holds row n, columns
-// 4h + 8q + t (q, t = 0..3) in out[4q + t].
+// Filter one patch held in x (lane layout) into z' in the same layout:
+// lane (h, n) holds row n, columns 4h + 8q + t (q, t = 0..3) in out[4q + t];
+// z' norm is the filtered patch over n^2 (patch_weight adds the triangular
+// weight).
 // SQRT: alpha = 0.5 (the usual Goldstein exponent), the weight is one
 // square root.
 template <bool SQRT>
 __device__ __forceinline__ void filter_patch_regs(const Consts &K, const SmoothLds &U,
                                                   const float2 x[16], int h, int n, float alpha,
-                                                  float2 out[16]) {
+                                                  float2 out[16], float &norm) {
   const int lane = 32 * h + n;
   float xr[16], xi[16];
   float mx = 0.f;
@@ -327,21 +328,18 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const SmoothL
     sm = mfma(t1l[c], u1h[c], sm);
     sm = mfma(t1h[c], u1h[c], sm);
   }
-  float s[16];
+  // S = l1 sm; the mean of S and S / mean with l1 applied once
   float lsum = 0.f;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    s[r] = sm[r] * kLam1;
-    lsum += s[r];
-  }
-  const float mean = wave_sum(lsum) * (1.0f / (kN * kN));
+  for (int r = 0; r < 16; ++r) lsum += sm[r];
+  const float mean = wave_sum(lsum) * (kLam1 / (kN * kN));
   // goldstein.cu:170-175 on the unscaled data: mean > 1e-8 (S carries s0)
   const bool ok = mean > 1e-8f * s0;
-  const float inv = ok ? 1.0f / mean : 0.f;
+  const float inv = ok ? kLam1 / mean : 0.f;
   float mw = 0.f;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    float sn = s[r] * inv;
+    float sn = sm[r] * inv;
     if (sn > 0.f) {  // powf(sn, alpha) > 0 only here; Z is kept elsewhere
       float w = SQRT ? __builtin_amdgcn_sqrtf(sn)
                      : __builtin_amdgcn_exp2f(alpha * __builtin_amdgcn_logf(sn));
@@ -363,19 +361,19 @@ __device__ __forceinline__ void filter_patch_regs(const Consts &K, const SmoothL
   // inverse transform of the rescaled spectrum: z' n^2 s0 s2
   cgemm_v<true>(zr, zi, K.F, yr, yi);
   cgemm_v<true>(yr, yi, K.F, zr, zi);
-  // weighted output (goldstein.cu:257-267), undo the patch scale
-  const float wy = (float)(n < kN / 2 ? n : kN - 1 - n);
   // z' / n^2 = result / (s0 s2 n^2) = result 2^(e + e2 - 2 kNormExp - 10)
-  const float norm = ldexpf(1.0f, e + e2 - 2 * kNormExp - 10);
+  norm = ldexpf(1.0f, e + e2 - 2 * kNormExp - 10);
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int col = 4 * h + 8 * q + t;
-      const float wx = (float)(col < kN / 2 ? col : kN - 1 - col);
-      const float w = (wx + wy) * norm;
-      out[4 * q + t] = make_float2(zr[4 * q + t] * w, zi[4 * q + t] * w);
-    }
+  for (int r = 0; r < 16; ++r) out[r] = make_float2(zr[r], zi[r]);
+}
+
+// Triangular output weight (goldstein.cu:257-267) of lane (h, n)'s entry
+// 4q + t times norm.
+__device__ __forceinline__ float patch_weight(int h, int n, int q, int t, float norm) {
+  const float wy = (float)(n < kN / 2 ? n : kN - 1 - n);
+  const int col = 4 * h + 8 * q + t;
+  const float wx = (float)(col < kN / 2 ? col : kN - 1 - col);
+  return fmaf(wx, norm, wy * norm);
 }
 
 }  // namespace mf
