@@ -101,6 +101,86 @@ __device__ __forceinline__ void coherence_tile(const float2 *__restrict__ ifg,
   __syncthreads();
 }
 
+// Windowed coherence over a strip of 64 columns x kStripRows rows (block
+// 64 x 4), for win <= 2 kStripMaxHalo + 1.  The window sums are separable:
+// the strip's input rows are read once, 4 at a time, into LDS; each row's
+// horizontal win-sums go into a ring of rows; each output pixel adds the
+// win row sums above and below it.  Zero outside the image.  Contains
+// block barriers (uniform); the last one lets the caller reuse L.
+constexpr int kStripRows = 32, kStripMaxHalo = 6, kStripRing = 16;
+static_assert(kStripRing >= kCohTY + 2 * kStripMaxHalo, "ring holds a window of rows");
+
+struct CohStrip {
+  float in[4][kCohTY][kCohTX + 2 * kStripMaxHalo];  // channels x 4 input rows x columns
+  float hs[4][kStripRing][kCohTX];                  // horizontal sums, ring of rows
+};
+
+__device__ __forceinline__ void coherence_strip(const float2 *__restrict__ ifg,
+                                                const float *__restrict__ p1,
+                                                const float *__restrict__ p2, int nr, int nc,
+                                                int win, int fill, uint32_t seed,
+                                                float *__restrict__ coh, int strip_x, int strip_y,
+                                                CohStrip &L) {
+  const int h = win / 2, w2 = 2 * h;
+  const int x = threadIdx.x, y = threadIdx.y;
+  const int c0 = strip_x * kCohTX, rb = strip_y * kStripRows;
+  const int re = min(rb + kStripRows, nr);
+  int nl = rb - h;  // next input row to load
+  for (int r0 = rb; r0 < re; r0 += kCohTY) {
+    // horizontal sums of the input rows up to r0 + kCohTY - 1 + h
+    while (nl <= r0 + kCohTY - 1 + h) {
+      const int rr = nl + y;
+      for (int i = x; i < kCohTX + w2; i += kCohTX) {
+        const int cc = c0 - h + i;
+        float a = 0.f, b = 0.f, u = 0.f, v = 0.f;
+        if (rr >= 0 && rr < nr && cc >= 0 && cc < nc) {
+          const size_t o = (size_t)rr * nc + cc;
+          const float2 z = ifg[o];
+          a = z.x;
+          b = z.y;
+          u = p1[o];
+          v = p2[o];
+        }
+        L.in[0][y][i] = a;
+        L.in[1][y][i] = b;
+        L.in[2][y][i] = u;
+        L.in[3][y][i] = v;
+      }
+      __syncthreads();
+      float sa = 0.f, sb = 0.f, su = 0.f, sv = 0.f;
+      for (int j = 0; j <= w2; ++j) {
+        sa += L.in[0][y][x + j];
+        sb += L.in[1][y][x + j];
+        su += L.in[2][y][x + j];
+        sv += L.in[3][y][x + j];
+      }
+      const int slot = rr & (kStripRing - 1);
+      L.hs[0][slot][x] = sa;
+      L.hs[1][slot][x] = sb;
+      L.hs[2][slot][x] = su;
+      L.hs[3][slot][x] = sv;
+      __syncthreads();
+      nl += kCohTY;
+    }
+    const int r = r0 + y, c = c0 + x;
+    if (r < re && c < nc) {
+      float ax = 0.f, ay = 0.f, a1 = 0.f, a2 = 0.f;
+      for (int i = 0; i <= w2; ++i) {
+        const int slot = (r - h + i) & (kStripRing - 1);
+        ax += L.hs[0][slot][x];
+        ay += L.hs[1][slot][x];
+        a1 += L.hs[2][slot][x];
+        a2 += L.hs[3][slot][x];
+      }
+      const float den = sqrtf(a1 * a2);
+      const float g = den > 0.f ? sqrtf(ax * ax + ay * ay) / den : 0.f;
+      const size_t o = (size_t)r * nc + c;
+      coh[o] = coh_fill(g, fill, seed, o);
+    }
+  }
+  __syncthreads();
+}
+
 // Look-cell estimator (coherence.py:142): |ifg| / (sqrt(p1)+eps) / (sqrt(p2)+eps).
 __device__ __forceinline__ float coherence_cell(float2 v, float p1, float p2) {
   float a1 = sqrtf(p1) + 1e-12f, a2 = sqrtf(p2) + 1e-12f;

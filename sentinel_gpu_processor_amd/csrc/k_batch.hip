@@ -253,28 +253,41 @@ __global__ __launch_bounds__(256) void kb_gather(const S1C PairDev *__restrict__
   }
 }
 
-// Coherence over 64 x 4 tiles: LDS-tiled window sums (win >= 3) or the
-// look-cell estimator (win = 1).
+// Coherence over strips of 64 x kStripRows: separable window sums
+// (coherence_strip), the strip's 64 x 4 LDS tiles for larger windows, or
+// the look-cell estimator (win = 1).
 __global__ __launch_bounds__(256) void kb_coherence(const S1C PairDev *__restrict__ P,
                                                     const S1C long *__restrict__ off, int n,
                                                     int win, int fill, uint32_t seed) {
-  __shared__ CohTile L;
+  __shared__ union {
+    CohTile t;
+    CohStrip s;
+  } L;
   const long total = off[n];
   int ph = -1;
   for (long t = blockIdx.x; t < total; t += gridDim.x) {
     const int p = ph = find_item(off, n, t, ph);
     const PairDev &d = *(const PairDev *)&P[p];
     const long lt = t - off[p];
-    const int tiles_x = (d.nc_ml + kCohTX - 1) / kCohTX;
-    const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
-    if (win == 1) {
-      const int c = tx * kCohTX + threadIdx.x, r = ty * kCohTY + threadIdx.y;
-      if (r < d.nr_ml && c < d.nc_ml) {
-        size_t o = (size_t)r * d.nc_ml + c;
-        d.coh[o] = coh_fill(coherence_cell(d.ifg[o], d.pr[o], d.ps[o]), fill, seed, o);
-      }
+    const int strips_x = (d.nc_ml + kCohTX - 1) / kCohTX;
+    const int sy = (int)(lt / strips_x), sx = (int)(lt - (long)sy * strips_x);
+    if (win > 1 && win / 2 <= kStripMaxHalo) {
+      coherence_strip(d.ifg, d.pr, d.ps, d.nr_ml, d.nc_ml, win, fill, seed, d.coh, sx, sy, L.s);
     } else {
-      coherence_tile(d.ifg, d.pr, d.ps, d.nr_ml, d.nc_ml, win, fill, seed, d.coh, tx, ty, L);
+      // the strip's 64 x 4 tiles
+      for (int ty = sy * (kStripRows / kCohTY); ty < (sy + 1) * (kStripRows / kCohTY); ++ty) {
+        if (ty * kCohTY >= d.nr_ml) break;
+        if (win == 1) {
+          const int c = sx * kCohTX + threadIdx.x, r = ty * kCohTY + threadIdx.y;
+          if (r < d.nr_ml && c < d.nc_ml) {
+            size_t o = (size_t)r * d.nc_ml + c;
+            d.coh[o] = coh_fill(coherence_cell(d.ifg[o], d.pr[o], d.ps[o]), fill, seed, o);
+          }
+        } else {
+          coherence_tile(d.ifg, d.pr, d.ps, d.nr_ml, d.nc_ml, win, fill, seed, d.coh, sx, ty,
+                         L.t);
+        }
+      }
     }
   }
 }
@@ -545,7 +558,8 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
     b->off[kStGold][p + 1] = b->off[kStGold][p] + (d.filt ? gitems : 0);
     long gtiles = (long)((d.nc_ml + 63) / 64) * ((d.nr_ml + 3) / 4);
     b->off[kStGather][p + 1] = b->off[kStGather][p] + (d.filt ? gtiles : 0);
-    long ctiles = (long)((d.nc_ml + kCohTX - 1) / kCohTX) * ((d.nr_ml + kCohTY - 1) / kCohTY);
+    long ctiles =
+        (long)((d.nc_ml + kCohTX - 1) / kCohTX) * ((d.nr_ml + kStripRows - 1) / kStripRows);
     b->off[kStCoh][p + 1] = b->off[kStCoh][p] + (d.coh ? ctiles : 0);
   }
   pair_prefix(b);

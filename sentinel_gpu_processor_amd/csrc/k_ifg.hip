@@ -133,6 +133,15 @@ __global__ __launch_bounds__(256) void k_coherence_tiled(
   coherence_tile(ifg, p1, p2, nr, nc, win, fill, seed, coh, blockIdx.x, blockIdx.y, L);
 }
 
+// Windowed coherence over 64 x kStripRows strips with separable window
+// sums (coherence_strip).
+__global__ __launch_bounds__(256) void k_coherence_strip(
+    const float2 *__restrict__ ifg, const float *__restrict__ p1, const float *__restrict__ p2,
+    int nr, int nc, int win, int fill, uint32_t seed, float *__restrict__ coh) {
+  __shared__ CohStrip L;
+  coherence_strip(ifg, p1, p2, nr, nc, win, fill, seed, coh, blockIdx.x, blockIdx.y, L);
+}
+
 }  // namespace
 
 extern "C" {
@@ -207,6 +216,13 @@ int s1ifg_coherence(s1ifg_ctx *ctx, const s1ifg_cf32 *ifg_ml, const float *pow_r
   if (fill < S1IFG_COH_FILL_CLAMP || fill > S1IFG_COH_FILL_HASH)
     return set_error(S1IFG_EINVAL, "fill rule %d", fill);
   dim3 blk(64, 4), grd((ncol + 63) / 64, (nrow + 3) / 4);
+  if (win > 1 && win / 2 <= kStripMaxHalo) {
+    dim3 sgrd((ncol + kCohTX - 1) / kCohTX, (nrow + kStripRows - 1) / kStripRows);
+    hipLaunchKernelGGL(k_coherence_strip, sgrd, blk, 0, ctx->stream, (const float2 *)ifg_ml,
+                       pow_ref, pow_sec, nrow, ncol, win, fill, seed, coh);
+    S1_LAUNCH_CHECK("k_coherence_strip");
+    return S1IFG_OK;
+  }
   if (win > 1 && win / 2 <= kCohMaxHalo) {
     hipLaunchKernelGGL(k_coherence_tiled, grd, blk, 0, ctx->stream, (const float2 *)ifg_ml,
                        pow_ref, pow_sec, nrow, ncol, win, fill, seed, coh);
