@@ -101,12 +101,14 @@ __device__ __forceinline__ void coherence_tile(const float2 *__restrict__ ifg,
   __syncthreads();
 }
 
-// Windowed coherence over a strip of 64 columns x kStripRows rows (block
-// 64 x 4), for win <= 2 kStripMaxHalo + 1.  The window sums are separable:
-// the strip's input rows are read once, 4 at a time, into LDS; each row's
-// horizontal win-sums go into a ring of rows; each output pixel adds the
-// win row sums above and below it.  Zero outside the image.  Contains
-// block barriers (uniform); the last one lets the caller reuse L.
+// Window sums of four channels (a complex, b, c) over a win x win window
+// centred on each pixel of a strip of 64 columns x kStripRows rows (block
+// 64 x 4), for win <= 2 kStripMaxHalo + 1, zero outside the image.  The
+// sums are separable: the strip's input rows are read once, 4 at a time,
+// into LDS; each row's horizontal win-sums go into a ring of rows; each
+// output pixel adds the win row sums above and below it, and f(r, c, sa,
+// sb, s1, s2) gets the pixel's sums.  Contains block barriers (uniform);
+// the last one lets the caller reuse L.
 constexpr int kStripRows = 32, kStripMaxHalo = 6, kStripRing = 16;
 static_assert(kStripRing >= kCohTY + 2 * kStripMaxHalo, "ring holds a window of rows");
 
@@ -115,12 +117,12 @@ struct CohStrip {
   float hs[4][kStripRing][kCohTX];                  // horizontal sums, ring of rows
 };
 
-__device__ __forceinline__ void coherence_strip(const float2 *__restrict__ ifg,
-                                                const float *__restrict__ p1,
-                                                const float *__restrict__ p2, int nr, int nc,
-                                                int win, int fill, uint32_t seed,
-                                                float *__restrict__ coh, int strip_x, int strip_y,
-                                                CohStrip &L) {
+template <typename F>
+__device__ __forceinline__ void window_sums_strip(const float2 *__restrict__ za,
+                                                  const float *__restrict__ zb,
+                                                  const float *__restrict__ zc, int nr, int nc,
+                                                  int win, int strip_x, int strip_y, CohStrip &L,
+                                                  F f) {
   const int h = win / 2, w2 = 2 * h;
   const int x = threadIdx.x, y = threadIdx.y;
   const int c0 = strip_x * kCohTX, rb = strip_y * kStripRows;
@@ -135,11 +137,11 @@ __device__ __forceinline__ void coherence_strip(const float2 *__restrict__ ifg,
         float a = 0.f, b = 0.f, u = 0.f, v = 0.f;
         if (rr >= 0 && rr < nr && cc >= 0 && cc < nc) {
           const size_t o = (size_t)rr * nc + cc;
-          const float2 z = ifg[o];
+          const float2 z = za[o];
           a = z.x;
           b = z.y;
-          u = p1[o];
-          v = p2[o];
+          u = zb[o];
+          v = zc[o];
         }
         L.in[0][y][i] = a;
         L.in[1][y][i] = b;
@@ -172,13 +174,26 @@ __device__ __forceinline__ void coherence_strip(const float2 *__restrict__ ifg,
         a1 += L.hs[2][slot][x];
         a2 += L.hs[3][slot][x];
       }
-      const float den = sqrtf(a1 * a2);
-      const float g = den > 0.f ? sqrtf(ax * ax + ay * ay) / den : 0.f;
-      const size_t o = (size_t)r * nc + c;
-      coh[o] = coh_fill(g, fill, seed, o);
+      f(r, c, ax, ay, a1, a2);
     }
   }
   __syncthreads();
+}
+
+// Windowed coherence of one strip (window_sums_strip).
+__device__ __forceinline__ void coherence_strip(const float2 *__restrict__ ifg,
+                                                const float *__restrict__ p1,
+                                                const float *__restrict__ p2, int nr, int nc,
+                                                int win, int fill, uint32_t seed,
+                                                float *__restrict__ coh, int strip_x, int strip_y,
+                                                CohStrip &L) {
+  window_sums_strip(ifg, p1, p2, nr, nc, win, strip_x, strip_y, L,
+                    [&](int r, int c, float ax, float ay, float a1, float a2) {
+                      const float den = sqrtf(a1 * a2);
+                      const float g = den > 0.f ? sqrtf(ax * ax + ay * ay) / den : 0.f;
+                      const size_t o = (size_t)r * nc + c;
+                      coh[o] = coh_fill(g, fill, seed, o);
+                    });
 }
 
 // Look-cell estimator (coherence.py:142): |ifg| / (sqrt(p1)+eps) / (sqrt(p2)+eps).

@@ -632,6 +632,49 @@ __global__ __launch_bounds__(kCohTX * kCohTY) void k_coh_bwd_box_tiled(
   if (g2) g2[o] = s2;
 }
 
+// The same two passes over 64 x kStripRows strips (window_sums_strip).
+__global__ __launch_bounds__(kCohTX * kCohTY) void k_coh_bwd_coef_strip(
+    const float2 *__restrict__ ifg, const float *__restrict__ p1, const float *__restrict__ p2,
+    int nr, int nc, int win, const float *__restrict__ g, float2 *__restrict__ cA,
+    float *__restrict__ cB, float *__restrict__ cC) {
+  __shared__ CohStrip L;
+  window_sums_strip(ifg, p1, p2, nr, nc, win, blockIdx.x, blockIdx.y, L,
+                    [&](int r, int c, float sx, float sy, float s1, float s2) {
+                      const size_t o = (size_t)r * nc + c;
+                      const float gq = g[o];
+                      float2 ra = make_float2(0.f, 0.f);
+                      float rb = 0.f, rc = 0.f;
+                      const float den = sqrtf(s1 * s2);
+                      const float mag = sqrtf(sx * sx + sy * sy);
+                      if (den > 0.f && mag > 0.f) {
+                        const float gam = mag / den;
+                        if (gam <= 1.0f) {
+                          const float s = gq / (mag * den);
+                          ra = make_float2(s * sx, s * sy);
+                          rb = -0.5f * gq * gam / s1;
+                          rc = -0.5f * gq * gam / s2;
+                        }
+                      }
+                      cA[o] = ra;
+                      cB[o] = rb;
+                      cC[o] = rc;
+                    });
+}
+
+__global__ __launch_bounds__(kCohTX * kCohTY) void k_coh_bwd_box_strip(
+    const float2 *__restrict__ cA, const float *__restrict__ cB, const float *__restrict__ cC,
+    int nr, int nc, int win, float2 *__restrict__ gI, float *__restrict__ g1,
+    float *__restrict__ g2) {
+  __shared__ CohStrip L;
+  window_sums_strip(cA, cB, cC, nr, nc, win, blockIdx.x, blockIdx.y, L,
+                    [&](int r, int c, float sx, float sy, float s1, float s2) {
+                      const size_t o = (size_t)r * nc + c;
+                      if (gI) gI[o] = make_float2(sx, sy);
+                      if (g1) g1[o] = s1;
+                      if (g2) g2[o] = s2;
+                    });
+}
+
 }  // namespace
 
 namespace s1ifg {
@@ -737,6 +780,16 @@ int s1ifg_coherence_backward(s1ifg_ctx *ctx, const s1ifg_cf32 *ifg_ml, const flo
   float2 *cA = (float2 *)ws;
   float *cB = (float *)((char *)ws + ((n * 8 + 255) & ~(size_t)255));
   float *cC = (float *)((char *)cB + ((n * 4 + 255) & ~(size_t)255));
+  if (win / 2 <= kStripMaxHalo) {
+    dim3 sgrd((ncol + kCohTX - 1) / kCohTX, (nrow + kStripRows - 1) / kStripRows);
+    hipLaunchKernelGGL(k_coh_bwd_coef_strip, sgrd, blk, 0, ctx->stream, (const float2 *)ifg_ml,
+                       pow_ref, pow_sec, nrow, ncol, win, g_coh, cA, cB, cC);
+    S1_LAUNCH_CHECK("k_coh_bwd_coef_strip");
+    hipLaunchKernelGGL(k_coh_bwd_box_strip, sgrd, blk, 0, ctx->stream, cA, cB, cC, nrow, ncol,
+                       win, (float2 *)g_ifg, g_pow_ref, g_pow_sec);
+    S1_LAUNCH_CHECK("k_coh_bwd_box_strip");
+    return S1IFG_OK;
+  }
   if (win / 2 <= kCohMaxHalo) {
     hipLaunchKernelGGL(k_coh_bwd_coef_tiled, grd, blk, 0, ctx->stream, (const float2 *)ifg_ml,
                        pow_ref, pow_sec, nrow, ncol, win, g_coh, cA, cB, cC);
