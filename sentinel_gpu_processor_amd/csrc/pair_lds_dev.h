@@ -28,6 +28,7 @@ namespace s1ifg {
 constexpr int kPairTile = 16;        // look cells per tile side
 constexpr int kPairWin = 4864;       // staged samples of burst 0 (38 KB; 4 blocks per CU)
 constexpr int kPairMaxPix = 4;       // target pixels per look cell on this path
+constexpr int kStage = 8;            // window loads in flight per thread while staging
 
 struct PairLds {
   float2 win[kPairWin];  // burst 0's tap window
@@ -141,10 +142,29 @@ __device__ __forceinline__ void pair_tile_sinc(
     const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
     const bool staged = any && W * H <= kPairWin;
     const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
-    if (staged)
-      for (int row = tid >> 6; row < H; row += 4)
-        for (int col = tid & 63; col < W; col += 64)
-          L.win[row * W + col] = z0[(size_t)(wa0 + row) * G0.samples + wr0 + col];
+    if (staged) {
+      // element i = row * W + col, 256 apart per thread; kStage loads are
+      // issued before their LDS stores (row and column advance by the
+      // quotient and remainder of 256 / W)
+      const int n = (int)(W * H), dq = 256 / wc, dr = 256 - dq * wc;
+      int row = tid / wc, col = tid - row * wc;
+      for (int i0 = tid; i0 < n; i0 += kStage * 256) {
+        float2 v[kStage];
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+          if (i0 + k * 256 < n) v[k] = z0[(size_t)(wa0 + row) * G0.samples + wr0 + col];
+          row += dq;
+          col += dr;
+          if (col >= wc) {
+            col -= wc;
+            ++row;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kStage; ++k)
+          if (i0 + k * 256 < n) L.win[i0 + k * 256] = v[k];
+      }
+    }
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < kPairMaxPix; ++p) {
