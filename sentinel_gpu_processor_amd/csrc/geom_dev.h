@@ -466,14 +466,26 @@ __device__ __forceinline__ bool tile_outside(const GeomDev &g, const OffPoly &op
   const int x0 = max(min(xa, xb) - 1, 0), x1 = min(max(xa, xb) + 2, dem.ncol - 1);
   const int nx = x1 - x0 + 1, cnt = (y1 - y0 + 1) * nx;
   if (cnt > kSkipDemMax) return false;
+  // (8 loads per lane in flight per step)
   float lo = INFINITY, hi = -INFINITY;
-  for (int k = lane; k < cnt; k += 64) {
-    const int y = y0 + k / nx, x = x0 + k - (k / nx) * nx;
-    const size_t o = (size_t)y * dem.ncol + x;
-    const float h = dem.dtype == S1IFG_DEM_I16 ? (float)((const short *)dem.data)[o]
-                                               : ((const float *)dem.data)[o];
-    lo = fminf(lo, h);
-    hi = fmaxf(hi, h);
+  for (int k0 = lane; k0 < cnt; k0 += 8 * 64) {
+    float v[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int k = k0 + 64 * m;
+      v[m] = NAN;
+      if (k < cnt) {
+        const int y = y0 + k / nx, x = x0 + k - (k / nx) * nx;
+        const size_t o = (size_t)y * dem.ncol + x;
+        v[m] = dem.dtype == S1IFG_DEM_I16 ? (float)((const short *)dem.data)[o]
+                                          : ((const float *)dem.data)[o];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      lo = fminf(lo, v[m]);
+      hi = fmaxf(hi, v[m]);
+    }
   }
   float dlo = 0.f, dhi = 0.f;
   if (dh) {
@@ -481,11 +493,22 @@ __device__ __forceinline__ bool tile_outside(const GeomDev &g, const OffPoly &op
     if (tn > kSkipDemMax) return false;
     dlo = INFINITY;
     dhi = -INFINITY;
-    for (int k = lane; k < tn; k += 64) {
-      const int y = r0 + k / tw, x = c0 + k - (k / tw) * tw;
-      const float v = dh[(size_t)y * ld + x];
-      dlo = fminf(dlo, v);
-      dhi = fmaxf(dhi, v);
+    for (int k0 = lane; k0 < tn; k0 += 8 * 64) {
+      float v[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int k = k0 + 64 * m;
+        v[m] = NAN;
+        if (k < tn) {
+          const int y = r0 + k / tw, x = c0 + k - (k / tw) * tw;
+          v[m] = dh[(size_t)y * ld + x];
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        dlo = fminf(dlo, v[m]);
+        dhi = fmaxf(dhi, v[m]);
+      }
     }
   }
   for (int m = 32; m >= 1; m >>= 1) {

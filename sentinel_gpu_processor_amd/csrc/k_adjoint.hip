@@ -306,6 +306,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_pair_bwd(
 // per burst (worst seen ~1,900) where 16 x 16 needed ~1,900 (~3,500).
 constexpr int kBwdRows = 8, kBwdCols = 32, kBwdThreads = kBwdRows * kBwdCols;
 constexpr int kBwdWin = 2048;
+constexpr int kBwdStage = 8;  // window loads in flight per thread while staging
 // Skip regions: 2 x 2 tiles (16 x 64 target pixels) share one skip flag per
 // burst, tested by k_bwd_flags before the tiles run.
 constexpr int kBwdRegion = 2;
@@ -424,10 +425,27 @@ __global__ __launch_bounds__(kBwdThreads) void k_pair_bwd_tiles(
     wr0[k] = mr - 3;
     wa0[k] = ma - 3;
     if (staged[k]) {
-      const int ns = G[k]->samples;
-      for (int row = tid >> 6; row < H; row += kBwdThreads / 64)
-        for (int col = tid & 63; col < W; col += 64)
-          L.win[k][row * W + col] = Z[k][(size_t)(wa0[k] + row) * ns + wr0[k] + col];
+      // element i = row * W + col, kBwdThreads apart per thread; kBwdStage
+      // loads are issued before their LDS stores
+      const int ns = G[k]->samples, w = wc[k];
+      const int n = (int)(W * H), dq = kBwdThreads / w, dr = kBwdThreads - dq * w;
+      int row = tid / w, col = tid - row * w;
+      for (int i0 = tid; i0 < n; i0 += kBwdStage * kBwdThreads) {
+        float2 v[kBwdStage];
+#pragma unroll
+        for (int m = 0; m < kBwdStage; ++m) {
+          if (i0 + m * kBwdThreads < n) v[m] = Z[k][(size_t)(wa0[k] + row) * ns + wr0[k] + col];
+          row += dq;
+          col += dr;
+          if (col >= w) {
+            col -= w;
+            ++row;
+          }
+        }
+#pragma unroll
+        for (int m = 0; m < kBwdStage; ++m)
+          if (i0 + m * kBwdThreads < n) L.win[k][i0 + m * kBwdThreads] = v[m];
+      }
     }
   }
   __syncthreads();
