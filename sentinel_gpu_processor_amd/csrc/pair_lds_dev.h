@@ -28,7 +28,7 @@ namespace s1ifg {
 constexpr int kPairTile = 16;        // look cells per tile side
 constexpr int kPairWin = 4864;       // staged samples of burst 0 (38 KB; 4 blocks per CU)
 constexpr int kPairMaxPix = 4;       // target pixels per look cell on this path
-constexpr int kStage = 8;            // window loads in flight per thread while staging
+constexpr int kStage = 16;           // window loads in flight per thread while staging
 
 struct PairLds {
   float2 win[kPairWin];  // burst 0's tap window
