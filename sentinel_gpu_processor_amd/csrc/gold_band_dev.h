@@ -212,18 +212,28 @@ __device__ __forceinline__ void band_sum_tile(const float2 *__restrict__ partial
     const float ws = L.cx[tx] * L.sy[tyi] + L.cy[tyi] * L.sx[tx];
     float2 r = make_float2(0.f, 0.f);
     if (ws > 0.f) {
-      float ax = 0.f, ay = 0.f;
+      // all covering partials loaded first, then added in (group, chunk)
+      // order
       const int nb = L.nb[tyi], nch = L.nch[tx];
-      for (int i = 0; i < nb; ++i) {
-        const size_t rowb = (size_t)L.grp[tyi][i] * nchunk;
-        const int t = L.ty[tyi][i];
-        for (int k = 0; k < nch; ++k) {
-          const float2 v =
-              partial[((rowb + L.ch[tx][k]) * kBandH + t) * kBandCols + L.xo[tx][k]];
-          ax += v.x;
-          ay += v.y;
+      float2 v[kSumGroups][2];
+#pragma unroll
+      for (int i = 0; i < kSumGroups; ++i)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          v[i][k] = make_float2(0.f, 0.f);
+          if (i < nb && k < nch)
+            v[i][k] = partial[(((size_t)L.grp[tyi][i] * nchunk + L.ch[tx][k]) * kBandH +
+                               L.ty[tyi][i]) * kBandCols + L.xo[tx][k]];
         }
-      }
+      float ax = 0.f, ay = 0.f;
+#pragma unroll
+      for (int i = 0; i < kSumGroups; ++i)
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+          if (i < nb && k < nch) {
+            ax += v[i][k].x;
+            ay += v[i][k].y;
+          }
       r = make_float2(ax / ws, ay / ws);
     }
     out[(size_t)y * ncol + x] = r;
