@@ -1,4 +1,4 @@
-// Fused burst pair with sinc interpolation, burst 0's tap windows staged
+// Fused burst pair with sinc interpolation, the bursts' tap windows staged
 // in LDS (shared by the per-pair and batched kernels).
 //
 // A 256-thread block owns a tile of 16 x 16 look cells.  Two wavefronts
@@ -6,17 +6,16 @@
 // (tile_outside: about half of a burst's target grid, the corners of its
 // latitude / longitude box, does); a burst that the tile misses is not
 // geocoded, and a tile that misses both writes zeros.  Each thread owns
-// one cell (rl * cl <= kPairMaxPix target pixels).  Burst 0: the thread
-// computes the FP64 geometry and phase of its pixels and keeps the
-// integer / fractional sampling positions and sin/cos of the phase in
-// registers; the block reduces the bounding box of the 8 x 8 tap windows,
-// loads that window of the deramped burst into LDS with row-contiguous
-// (coalesced) loads, and every pixel is interpolated from LDS (a window
-// larger than kPairWin samples is read from global memory instead).
-// Burst 1: pixel by pixel, geometry and interpolation straight from global
-// memory, so its taps go through the vector L1 while burst 0's go through
-// LDS.  All ways run the same arithmetic in the same order as
-// pair_ml_pixel, so the products are bit-identical to it.
+// one cell (rl * cl <= kPairMaxPix target pixels).  Then, for burst 0 and
+// then burst 1 through the same LDS window: the thread computes the FP64
+// geometry and phase of its pixels and keeps the integer / fractional
+// sampling positions and sin/cos of the phase in registers; the block
+// reduces the bounding box of the 8 x 8 tap windows, loads that window of
+// the deramped burst into LDS (stage_window: many loads in flight per
+// thread), and every pixel is interpolated from LDS (a window larger than
+// kPairWin samples is read from global memory instead).  All ways run the
+// same arithmetic in the same order as pair_ml_pixel, so the products are
+// bit-identical to it.
 #pragma once
 
 #include <climits>
@@ -31,10 +30,38 @@ constexpr int kPairMaxPix = 4;       // target pixels per look cell on this path
 constexpr int kStage = 16;           // window loads in flight per thread while staging
 
 struct PairLds {
-  float2 win[kPairWin];  // burst 0's tap window
-  int box[4];            // burst 0: min intr, min inta, max intr, max inta
+  float2 win[kPairWin];  // one burst's tap window at a time
+  int box[2][4];         // per burst: min intr, min inta, max intr, max inta
   int out[2];            // per burst: every pixel of the tile outside the burst
 };
+
+// Stage rows [wa0, wa0 + H) x columns [wr0, wr0 + W) of burst z (row
+// stride ns) into win (row-major, W per row).  All 256 threads call it;
+// element i = row * W + col is 256 apart per thread and kStage loads are
+// issued before their LDS stores (row and column advance by the quotient
+// and remainder of 256 / W).
+__device__ __forceinline__ void stage_window(const float2 *__restrict__ z, int ns, int wa0,
+                                             int wr0, int W, int H, float2 *win) {
+  const int tid = threadIdx.x;
+  const int n = W * H, dq = 256 / W, dr = 256 - dq * W;
+  int row = tid / W, col = tid - row * W;
+  for (int i0 = tid; i0 < n; i0 += kStage * 256) {
+    float2 v[kStage];
+#pragma unroll
+    for (int k = 0; k < kStage; ++k) {
+      if (i0 + k * 256 < n) v[k] = z[(size_t)(wa0 + row) * ns + wr0 + col];
+      row += dq;
+      col += dr;
+      if (col >= W) {
+        col -= W;
+        ++row;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kStage; ++k)
+      if (i0 + k * 256 < n) win[i0 + k * 256] = v[k];
+  }
+}
 
 __device__ __forceinline__ int wave_min_i(int v) {
   for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o));
@@ -59,7 +86,7 @@ __device__ __forceinline__ void pair_tile_sinc(
   const int rm = tile_y * kPairTile + (tid / kPairTile);
   const bool active = cm < nc_ml && rm < nr_ml;
   const int np = rl * cl;
-  if (tid < 4) L.box[tid] = tid < 2 ? INT_MAX : INT_MIN;
+  if (tid < 8) L.box[tid >> 2][tid & 3] = (tid & 3) < 2 ? INT_MAX : INT_MIN;
   // wavefront b < 2: is the whole tile outside burst b?
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (wave < 2) {
@@ -129,42 +156,20 @@ __device__ __forceinline__ void pair_tile_sinc(
       const int w0 = wave_min_i(lo_r), w1 = wave_min_i(lo_a);
       const int w2 = wave_max_i(hi_r), w3 = wave_max_i(hi_a);
       if ((tid & 63) == 0) {
-        atomicMin(&L.box[0], w0);
-        atomicMin(&L.box[1], w1);
-        atomicMax(&L.box[2], w2);
-        atomicMax(&L.box[3], w3);
+        atomicMin(&L.box[0][0], w0);
+        atomicMin(&L.box[0][1], w1);
+        atomicMax(&L.box[0][2], w2);
+        atomicMax(&L.box[0][3], w3);
       }
     }
     __syncthreads();
     // stage the tap window (columns min-3 .. max+4, rows likewise)
-    const int mr = L.box[0], ma = L.box[1], xr = L.box[2], xa = L.box[3];
+    const int mr = L.box[0][0], ma = L.box[0][1], xr = L.box[0][2], xa = L.box[0][3];
     const bool any = mr <= xr;
     const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
     const bool staged = any && W * H <= kPairWin;
     const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
-    if (staged) {
-      // element i = row * W + col, 256 apart per thread; kStage loads are
-      // issued before their LDS stores (row and column advance by the
-      // quotient and remainder of 256 / W)
-      const int n = (int)(W * H), dq = 256 / wc, dr = 256 - dq * wc;
-      int row = tid / wc, col = tid - row * wc;
-      for (int i0 = tid; i0 < n; i0 += kStage * 256) {
-        float2 v[kStage];
-#pragma unroll
-        for (int k = 0; k < kStage; ++k) {
-          if (i0 + k * 256 < n) v[k] = z0[(size_t)(wa0 + row) * G0.samples + wr0 + col];
-          row += dq;
-          col += dr;
-          if (col >= wc) {
-            col -= wc;
-            ++row;
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < kStage; ++k)
-          if (i0 + k * 256 < n) L.win[i0 + k * 256] = v[k];
-      }
-    }
+    if (staged) stage_window(z0, G0.samples, wa0, wr0, wc, (int)H, L.win);
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < kPairMaxPix; ++p) {
@@ -179,17 +184,19 @@ __device__ __forceinline__ void pair_tile_sinc(
       }
     }
   }
-  // Burst 1, pixel by pixel: geometry, interpolation from global memory
-  // (vector L1: the two memory paths share the taps), and the look means
-  // in the order of pair_ml_pixel.
-  float ax = 0.f, ay = 0.f, apr = 0.f, aps = 0.f;
-  float sx = 0.f, sy = 0.f, spr = 0.f, sps = 0.f;
-  const float inv_cl = 1.0f / cl, inv_rl = 1.0f / rl;
+  // Burst 1: geometry, its tap window in LDS once every thread is done
+  // with burst 0's, interpolation
+  float2 v1[kPairMaxPix];
+  {
+    int ir[kPairMaxPix], ia[kPairMaxPix];
+    float fr[kPairMaxPix], fa[kPairMaxPix], sn[kPairMaxPix], cs[kPairMaxPix];
+    unsigned vmask = 0;
+    int lo_r = INT_MAX, lo_a = INT_MAX, hi_r = INT_MIN, hi_a = INT_MIN;
 #pragma unroll
-  for (int p = 0; p < kPairMaxPix; ++p) {
-    if (p < np) {
-      float2 c = make_float2(0.f, 0.f);
-      if (active && !out1) {
+    for (int p = 0; p < kPairMaxPix; ++p) {
+      ir[p] = ia[p] = 0;
+      fr[p] = fa[p] = sn[p] = cs[p] = 0.f;
+      if (p < np && active && !out1) {
         const int i = p / cl, j = p - i * cl;
         double X[3];
         target_xyz(rows[rm * rl + i], cols[cm * cl + j], hp[p], X);
@@ -197,11 +204,59 @@ __device__ __forceinline__ void pair_tile_sinc(
         float s, cc;
         if (geocode_geom<S1IFG_INTERP_SINC>(G1, op1, X, rg, az, rng, s, cc)) {
           const int r0 = (int)floor(rg), a0 = (int)floor(az);
-          c = sinc_interp(z1 + (size_t)(a0 - 3) * G1.samples + (r0 - 3), G1.samples,
-                          (float)(rg - r0), (float)(az - a0), s, cc);
+          ir[p] = r0;
+          ia[p] = a0;
+          fr[p] = (float)(rg - r0);
+          fa[p] = (float)(az - a0);
+          sn[p] = s;
+          cs[p] = cc;
+          vmask |= 1u << p;
+          lo_r = min(lo_r, r0);
+          hi_r = max(hi_r, r0);
+          lo_a = min(lo_a, a0);
+          hi_a = max(hi_a, a0);
         }
       }
-      const float2 a = v0[p];
+    }
+    {
+      const int w0 = wave_min_i(lo_r), w1 = wave_min_i(lo_a);
+      const int w2 = wave_max_i(hi_r), w3 = wave_max_i(hi_a);
+      if ((tid & 63) == 0) {
+        atomicMin(&L.box[1][0], w0);
+        atomicMin(&L.box[1][1], w1);
+        atomicMax(&L.box[1][2], w2);
+        atomicMax(&L.box[1][3], w3);
+      }
+    }
+    __syncthreads();  // burst 0's window read by every thread, burst 1's box complete
+    const int mr = L.box[1][0], ma = L.box[1][1], xr = L.box[1][2], xa = L.box[1][3];
+    const bool any = mr <= xr;
+    const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
+    const bool staged = any && W * H <= kPairWin;
+    const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
+    if (staged) stage_window(z1, G1.samples, wa0, wr0, wc, (int)H, L.win);
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < kPairMaxPix; ++p) {
+      v1[p] = make_float2(0.f, 0.f);
+      if (p < np && (vmask & (1u << p))) {
+        if (staged)
+          v1[p] = sinc_interp(&L.win[(ia[p] - 3 - wa0) * wc + (ir[p] - 3 - wr0)], wc, fr[p],
+                              fa[p], sn[p], cs[p]);
+        else
+          v1[p] = sinc_interp(z1 + (size_t)(ia[p] - 3) * G1.samples + (ir[p] - 3), G1.samples,
+                              fr[p], fa[p], sn[p], cs[p]);
+      }
+    }
+  }
+  // the look means in the order of pair_ml_pixel
+  float ax = 0.f, ay = 0.f, apr = 0.f, aps = 0.f;
+  float sx = 0.f, sy = 0.f, spr = 0.f, sps = 0.f;
+  const float inv_cl = 1.0f / cl, inv_rl = 1.0f / rl;
+#pragma unroll
+  for (int p = 0; p < kPairMaxPix; ++p) {
+    if (p < np) {
+      const float2 a = v0[p], c = v1[p];
       sx += a.x * c.x + a.y * c.y;
       sy += a.x * c.y - a.y * c.x;
       spr += a.x * a.x + a.y * a.y;
