@@ -3,6 +3,7 @@
 #pragma once
 
 #include "s1ifg_internal.h"
+#include "wave_dev.h"
 
 namespace s1ifg {
 
@@ -511,12 +512,12 @@ __device__ __forceinline__ bool tile_outside(const GeomDev &g, const OffPoly &op
       }
     }
   }
-  for (int m = 32; m >= 1; m >>= 1) {
-    lo = fminf(lo, __shfl_xor(lo, m));
-    hi = fmaxf(hi, __shfl_xor(hi, m));
-    dlo = fminf(dlo, __shfl_xor(dlo, m));
-    dhi = fmaxf(dhi, __shfl_xor(dhi, m));
-  }
+  const auto fmin = [](float a, float b) { return fminf(a, b); };
+  const auto fmax = [](float a, float b) { return fmaxf(a, b); };
+  lo = wave_reduce(lo, fmin);
+  hi = wave_reduce(hi, fmax);
+  dlo = wave_reduce(dlo, fmin);
+  dhi = wave_reduce(dhi, fmax);
   // corner q = lane & 7: bit 0 column, bit 1 row, bit 2 height
   const int q = lane & 7;
   double X[3];

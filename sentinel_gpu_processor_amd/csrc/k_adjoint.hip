@@ -317,48 +317,16 @@ struct BwdLds {
   double red[kBwdThreads / 64][6];
 };
 
-// FP64 wavefront sum with the result in every lane, on the VALU: DPP
-// within rows of 16 lanes (quad permutations, then the half-row and row
-// mirrors), then the gfx950 lane swaps between rows (permlane16_swap) and
-// halves (permlane32_swap), each on the two 32-bit halves.  Fixed order.
-template <int CTRL>
-__device__ __forceinline__ double dpp_mov_d(double x) {
-  const uint2 u = __builtin_bit_cast(uint2, x);
-  uint2 r;
-  r.x = (unsigned)__builtin_amdgcn_update_dpp(0, (int)u.x, CTRL, 0xF, 0xF, false);
-  r.y = (unsigned)__builtin_amdgcn_update_dpp(0, (int)u.y, CTRL, 0xF, 0xF, false);
-  return __builtin_bit_cast(double, r);
-}
-
+// FP64 wavefront sum with the result in every lane (fixed order).
 __device__ __forceinline__ double wave_sum_d(double v) {
-  v += dpp_mov_d<0xB1>(v);   // quad_perm [1, 0, 3, 2]
-  v += dpp_mov_d<0x4E>(v);   // quad_perm [2, 3, 0, 1]
-  v += dpp_mov_d<0x141>(v);  // row_half_mirror
-  v += dpp_mov_d<0x140>(v);  // row_mirror
-  {
-    const uint2 u = __builtin_bit_cast(uint2, v);
-    const auto lo = __builtin_amdgcn_permlane16_swap(u.x, u.x, false, false);
-    const auto hi = __builtin_amdgcn_permlane16_swap(u.y, u.y, false, false);
-    v = __builtin_bit_cast(double, make_uint2((unsigned)lo[0], (unsigned)hi[0])) +
-        __builtin_bit_cast(double, make_uint2((unsigned)lo[1], (unsigned)hi[1]));
-  }
-  {
-    const uint2 u = __builtin_bit_cast(uint2, v);
-    const auto lo = __builtin_amdgcn_permlane32_swap(u.x, u.x, false, false);
-    const auto hi = __builtin_amdgcn_permlane32_swap(u.y, u.y, false, false);
-    v = __builtin_bit_cast(double, make_uint2((unsigned)lo[0], (unsigned)hi[0])) +
-        __builtin_bit_cast(double, make_uint2((unsigned)lo[1], (unsigned)hi[1]));
-  }
-  return v;
+  return wave_reduce(v, [](double a, double b) { return a + b; });
 }
 
 __device__ __forceinline__ int wmin_i(int v) {
-  for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o));
-  return v;
+  return wave_reduce(v, [](int a, int b) { return min(a, b); });
 }
 __device__ __forceinline__ int wmax_i(int v) {
-  for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
-  return v;
+  return wave_reduce(v, [](int a, int b) { return max(a, b); });
 }
 
 // Skip flags of the backward's regions (2 x 2 tiles): wavefront w of the

@@ -21,6 +21,7 @@
 #include <climits>
 
 #include "geom_dev.h"
+#include "wave_dev.h"
 
 namespace s1ifg {
 
@@ -63,26 +64,12 @@ __device__ __forceinline__ void stage_window(const float2 *__restrict__ z, int n
   }
 }
 
-// Integer wavefront min / max with the result in every lane, on the VALU:
-// DPP within rows of 16 lanes (quad permutations, then the half-row and row
-// mirrors), then the gfx950 lane swaps between rows (permlane16_swap) and
-// halves (permlane32_swap).
-template <typename Op>
-__device__ __forceinline__ int wave_reduce_i(int v, Op op) {
-  v = op(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
-  v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
-  v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
-  v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false));  // row_mirror
-  const auto r16 = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
-  v = op((int)r16[0], (int)r16[1]);
-  const auto r32 = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
-  return op((int)r32[0], (int)r32[1]);
-}
+// Integer wavefront min / max with the result in every lane.
 __device__ __forceinline__ int wave_min_i(int v) {
-  return wave_reduce_i(v, [](int a, int b) { return min(a, b); });
+  return wave_reduce(v, [](int a, int b) { return min(a, b); });
 }
 __device__ __forceinline__ int wave_max_i(int v) {
-  return wave_reduce_i(v, [](int a, int b) { return max(a, b); });
+  return wave_reduce(v, [](int a, int b) { return max(a, b); });
 }
 
 // One tile (tile_y, tile_x) of look cells.  All 256 threads of the block
