@@ -668,7 +668,7 @@ def main():
                 unit="GB/s", frac=k["hbm_frac"], traffic=k["traffic"],
                 alg_bytes_per_launch=sb[dom], valu_issue_busy=k["valu_issue_busy"],
                 limiter=("latency and issue: per-pixel FP64 geometry chains, and 2 x 64 sinc "
-                         "taps per pixel through LDS (burst 0) and vector-L1 gathers (burst 1)"
+                         "taps per pixel from the tile's LDS window, one burst after the other"
                          if dom in ("pair", "geocode") else None),
                 kernels=kernels)
     if not args.no_calibrate_copy:
