@@ -49,6 +49,7 @@ def parse():
     p.add_argument("--iw-steps", type=int, default=5)
     p.add_argument("--no-adjoint", action="store_true", help="skip the C2 forward+backward")
     p.add_argument("--no-stack", action="store_true", help="skip the 20-date stack")
+    p.add_argument("--no-stitch", action="store_true", help="skip the IW subswath stitch")
     p.add_argument("--stack-dates", type=int, default=20)
     p.add_argument("--gold-mode", default="auto", choices=["auto", "fp32", "fp16_split"])
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -387,6 +388,97 @@ def run_stack(args, gold_mode):
     return res
 
 
+STITCH_SHAPE = dict(bursts=9, rows=1500, cols=6250, overlap=150)
+
+
+def stitch_inputs():
+    """Nine complex64 burst interferograms of 1500 x 6250 look cells (the C2
+    look grid) stacked in azimuth with 150-row overlaps: a smooth phase
+    field, a different phase offset per burst (each burst from its own
+    scene pair), unit-mean amplitude and phase noise (0.3 rad), 1 % empty
+    pixels.  Returns [(left, top, right, bottom, tensor, estimate)]."""
+    import torch
+    sh = STITCH_SHAPE
+    h, w, ov = sh["rows"], sh["cols"], sh["overlap"]
+    g = torch.Generator(device="cuda:0").manual_seed(7)
+    out = []
+    cols = torch.arange(w, device="cuda:0", dtype=torch.float32)[None, :]
+    for k in range(sh["bursts"]):
+        t = k * (h - ov)
+        rows = torch.arange(t, t + h, device="cuda:0", dtype=torch.float32)[:, None]
+        ph = 2e-3 * rows + 3e-3 * cols + 0.7 * k
+        ph = ph + 0.3 * torch.randn((h, w), generator=g, device="cuda:0")
+        amp = 0.5 + torch.rand((h, w), generator=g, device="cuda:0")
+        amp[torch.rand((h, w), generator=g, device="cuda:0") < 0.01] = 0
+        z = torch.polar(amp, ph).to(torch.complex64).contiguous()
+        out.append((0, t, w, t + h, z, True))
+    return out
+
+
+def run_stitch(args):
+    """IW subswath mosaic on the device (interfere.py:81-150): the nine
+    bursts of stitch_inputs(), every scene boundary estimated (mean + median
+    of the overlap's phase differences)."""
+    import torch
+
+    from sentinel_gpu_processor_amd import kernels as K
+    from sentinel_gpu_processor_amd.runtime import context
+    ctx = context()
+    bursts = stitch_inputs()
+    sh = STITCH_SHAPE
+    n, h, w, ov = sh["bursts"], sh["rows"], sh["cols"], sh["overlap"]
+    for _ in range(2):
+        K.stitch_subswath(ctx, bursts)
+    torch.cuda.synchronize()
+    steps = max(args.steps, 5)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        mos, _ = K.stitch_subswath(ctx, bursts)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    P, M, O = h * w, mos.numel(), ov * w
+    # compulsory traffic: burst 0 read and the mosaic written once; per later
+    # burst its pixels and the mosaic window read, its new pixels written,
+    # and the overlap read once more for the offset estimate
+    alg = 8 * (P + M + (n - 1) * (3 * P + O))
+    gbs = alg / (ms * 1e-3) / 1e9
+    r = dict(workload=f"{n} bursts of {h} x {w} complex64 look cells, {ov}-row overlaps, "
+                      f"every boundary estimated",
+             ms=round(ms, 4), mpix_s=round(n * P / (ms * 1e-3) / 1e6, 1),
+             alg_bytes=alg, gbs=round(gbs, 1), hbm_frac=round(gbs / 8000.0, 3))
+    print(f"[bench] stitch: {ms:.3f} ms per subswath ({r['mpix_s']} Mpix/s, {gbs:.0f} GB/s "
+          f"compulsory)", file=sys.stderr, flush=True)
+    if not args.no_cpu_baseline:
+        try:
+            r["cpu_baseline"] = cpu_baseline_stitch(bursts, mos)
+        except Exception as e:  # reported, never fatal
+            r["cpu_baseline"] = dict(error=repr(e))
+    return r
+
+
+def cpu_baseline_stitch(bursts, mos):
+    """The oracle's NumPy stitch (a port of interfere.py:81-150) on the same
+    subswath, one host thread; also the largest difference from the device
+    mosaic."""
+    import numpy as np
+
+    from oracle import ifg as OI
+    host = [((0, 0, l, t, r, b), z.cpu().numpy()) for (l, t, r, b, z, _) in bursts]
+    ids = [("s", str(k)) for k in range(len(host))]
+    t0 = time.perf_counter()
+    want, _, _ = OI.stitch_subswath(host, ids)
+    sec = time.perf_counter() - t0
+    diff = float(np.abs(mos.cpu().numpy() - want).max())
+    P = sum(z.size for _, z in host)
+    print(f"[bench] stitch cpu (oracle, 1 thread): {sec * 1e3:.0f} ms; max |gpu - cpu| {diff:.2e}",
+          file=sys.stderr, flush=True)
+    return dict(value=round(P / sec / 1e6, 2), unit="Mpix/s", cores=1, kind="port",
+                sample="the whole subswath", ms=round(sec * 1e3, 1), max_abs_diff=diff)
+
+
 def time_form(fn, steps, warmup, name):
     import torch
     for _ in range(warmup):
@@ -685,6 +777,12 @@ def main():
             stack = run_stack(args, res["_gold_mode"])
         except Exception as e:  # reported; the one-pair number stands alone
             stack = dict(error=repr(e))
+    stitch = None
+    if not args.no_stitch:
+        try:
+            stitch = run_stitch(args)
+        except Exception as e:  # reported; the one-pair number stands alone
+            stitch = dict(error=repr(e))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -715,6 +813,7 @@ def main():
         "iw": iw,
         "adjoint": res.get("_adjoint"),
         "stack": stack,
+        "stitch": stitch,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

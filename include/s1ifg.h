@@ -334,6 +334,38 @@ int s1ifg_batch_get_opts(const s1ifg_batch *b, s1ifg_batch_opts *opts);
  * the launch shape. */
 int s1ifg_batch_set_launch(s1ifg_batch *b, const s1ifg_batch_opts *opts);
 
+/* ---- Stitching (interfere.py:81-180) ---- */
+
+/* One burst interferogram of a subswath: device data of (bottom - top) rows
+ * x (right - left) columns, row-major, cf32 (or f32 phase when out_float),
+ * with its bounds on the full image grid. */
+typedef struct {
+  const void *data;
+  int32_t left, top, right, bottom;
+  int32_t estimate;  /* 1: this burst's ids differ from the previous burst's
+                        (interfere.py:91-95, 112-117): estimate the offset */
+  int32_t pad_;
+} s1ifg_stitch_burst;
+
+/* Mosaic of one subswath's bursts, already sorted by top + bottom
+ * (sario.py:240-245), into `mosaic` ((bottom - top) x (right - left) of the
+ * given subswath bounds, the first / last bursts' box, sario.py:230-237).
+ * Burst 0 is resampled into the bounds; each later burst, which must lie
+ * inside them, gets the running phase offset (angle of the mean phase
+ * difference over its overlap with the mosaic, plus the median of the
+ * residual angles) and fills the pixels still without data
+ * (interfere.py:97-143).  offsets: device array of n doubles receiving the
+ * offset applied to each burst (0 for burst 0), or NULL.  Replaces
+ * stitch_subswath (interfere.py:81-150) for in-memory bursts. */
+int s1ifg_stitch_subswath(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n,
+                          int out_float, void *mosaic, int left, int top, int right,
+                          int bottom, double *offsets);
+/* dst[r, c] = src[r, c] where src has data (real part, or value, non-zero):
+ * one subswath mosaic into the full image at dst (row stride ld_dst
+ * elements).  Replaces the merge loop of stitch (interfere.py:170-179). */
+int s1ifg_merge_subswath(s1ifg_ctx *ctx, const void *src, int nrow, int ncol, int out_float,
+                         void *dst, int64_t ld_dst);
+
 #ifdef __cplusplus
 }
 #endif

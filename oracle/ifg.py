@@ -155,45 +155,73 @@ def match_bursts(ref_bursts, sec_bursts):
     return out
 
 
-def stitch_subswath(bursts, ids):
+def stitch_subswath(bursts, ids, out_float=False):
     """interfere.py:81-150 on in-memory bursts.
 
-    bursts: list of (header6, data complex64) already sorted by top+bottom;
-    ids: list of (unique_idx1, unique_idx2) per burst.  Returns the stitched
-    array covering the subswath bounds (first/last burst) and the bounds.
+    bursts: list of (header6, data) already sorted by top+bottom, data
+    complex64 (out_float False) or float32 phase (out_float True); ids: list
+    of (unique_idx1, unique_idx2) per burst.  Returns the stitched array
+    covering the subswath bounds (first/last burst), the bounds, and the
+    phase offset applied to each burst (the running mean_phase_diff).
     """
+    dtype = np.float32 if out_float else np.complex64
     hs = [b[0] for b in bursts]
     left = min(hs[0][2], hs[-1][2])
     top = min(hs[0][3], hs[-1][3])
     right = max(hs[0][4], hs[-1][4])
     bottom = max(hs[0][5], hs[-1][5])
-    out = np.zeros((bottom - top, right - left), np.complex64)
+    out = np.zeros((bottom - top, right - left), dtype)
     mean_phase_diff = 0.0
     prev = None
+    offsets = []
     for i, ((h, data), uid) in enumerate(zip(bursts, ids)):
-        new_full = np.zeros((bottom - top, right - left), np.complex64)
         if i == 0:
             # CroppedImage.load_data -> resample into the subswath bounds
-            _paste(new_full, h, data, left, top, right, bottom)
-            out = new_full
+            _paste(out, h, data, left, top, right, bottom)
             prev = uid
+            offsets.append(0.0)
             continue
         old = out[h[3] - top: h[5] - top, h[2] - left: h[4] - left]
         new = data
-        replace = (old.real == 0) & (new.real != 0)
+        if out_float:
+            replace = (old == 0) & (new != 0)
+        else:
+            replace = (old.real == 0) & (new.real != 0)
         ovm = None
         if uid[0] != prev[0] or uid[1] != prev[1]:
-            ovm = (old.real != 0) & (new != 0)
+            ovm = (old != 0) & (new != 0) if out_float else (old.real != 0) & (new != 0)
         prev = uid
         if ovm is not None and np.any(ovm):
-            pdiff = np.conj(old[ovm]) * new[ovm]
+            if out_float:
+                pdiff = np.exp(1j * (-old[ovm] + new[ovm]))
+            else:
+                pdiff = np.conj(old[ovm]) * new[ovm]
             mean_phase_diff = np.angle(np.mean(pdiff))
             pdiff = pdiff * np.exp(-1j * mean_phase_diff)
             mean_phase_diff += np.median(np.angle(pdiff))
         if mean_phase_diff != 0:
-            new = new * np.exp(-1j * mean_phase_diff)
+            if out_float:
+                new = new - mean_phase_diff
+                new[new > np.pi] -= 2 * np.pi
+                new[new < -np.pi] += 2 * np.pi
+            else:
+                new = new * np.exp(-1j * mean_phase_diff)
         old[replace] = new[replace]
-    return out, (left, top, right, bottom)
+        offsets.append(float(mean_phase_diff))
+    return out, (left, top, right, bottom), offsets
+
+
+def merge_subswaths(subs, nrow0, ncol0, out_float=False):
+    """interfere.py:153-180 in memory: subswath mosaics (bounds4, data) into
+    one (nrow0, ncol0) image; later subswaths overwrite where their value
+    (real part) is non-zero."""
+    dtype = np.float32 if out_float else np.complex64
+    mm = np.zeros((nrow0, ncol0), dtype)
+    for (left, top, right, bottom), data in subs:
+        old = mm[top:bottom, left:right]
+        rep = (data != 0) if out_float else (data.real != 0)
+        old[rep] = data[rep]
+    return mm
 
 
 def _paste(dst, h, data, left, top, right, bottom):

@@ -91,6 +91,12 @@ class BatchOpts(ctypes.Structure):
                 ("stream_blocks_per_cu", ctypes.c_int32)]
 
 
+class StitchBurst(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("left", ctypes.c_int32), ("top", ctypes.c_int32),
+                ("right", ctypes.c_int32), ("bottom", ctypes.c_int32),
+                ("estimate", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
 # a *_blocks_per_cu value: one block per work item instead of persistent blocks
 BLOCK_PER_ITEM = -1
 
@@ -140,6 +146,8 @@ SIGNATURES = {
     "s1ifg_batch_bytes": [_P, ctypes.POINTER(_I64)],
     "s1ifg_batch_get_opts": [_P, ctypes.POINTER(BatchOpts)],
     "s1ifg_batch_set_launch": [_P, ctypes.POINTER(BatchOpts)],
+    "s1ifg_stitch_subswath": [_P, ctypes.POINTER(StitchBurst), _I, _I, _P, _I, _I, _I, _I, _P],
+    "s1ifg_merge_subswath": [_P, _P, _I, _I, _I, _P, _I64],
 }
 _VOID = {"s1ifg_ctx_destroy", "s1ifg_geom_destroy", "s1ifg_batch_destroy"}
 _STR = {"s1ifg_last_error"}

@@ -138,6 +138,55 @@ def coherence(ctx, ifg_ml, pow_ref, pow_sec, win=1, fill="clamp", seed=0, out=No
     return coh
 
 
+def stitch_subswath(ctx, bursts, out_float=False, bounds=None, offsets=False):
+    """Mosaic of one subswath's burst interferograms on the device
+    (interfere.py:81-150).
+
+    bursts: (left, top, right, bottom, data, estimate) per burst, sorted by
+    top + bottom; data complex64 (float32 phase when out_float) of the
+    burst's bounds; estimate: its ids differ from the previous burst's.
+    bounds: the subswath box, default the first / last bursts' box
+    (sario.py:230-237).  Returns (mosaic, (left, top, right, bottom)) and,
+    with offsets=True, the float64 phase offset applied to each burst."""
+    if not bursts:
+        raise ValueError("no bursts")
+    dt = torch.float32 if out_float else torch.complex64
+    if bounds is None:
+        f, l_ = bursts[0], bursts[-1]
+        bounds = (min(f[0], l_[0]), min(f[1], l_[1]), max(f[2], l_[2]), max(f[3], l_[3]))
+    left, top, right, bottom = (int(v) for v in bounds)
+    arr = (N.StitchBurst * len(bursts))()
+    dev = None
+    for i, (bl, bt, br, bb, data, est) in enumerate(bursts):
+        _need(data, dt, (int(bb) - int(bt), int(br) - int(bl)), f"burst {i}")
+        dev = data.device
+        arr[i] = N.StitchBurst(N.ptr(data), int(bl), int(bt), int(br), int(bb), int(bool(est)), 0)
+    mosaic = _new((bottom - top, right - left), dt, dev)
+    offs = _new((len(bursts),), torch.float64, dev) if offsets else None
+    N.check(ctx.lib.s1ifg_stitch_subswath(ctx.handle, arr, len(bursts), int(bool(out_float)),
+                                          N.ptr(mosaic), left, top, right, bottom,
+                                          N.ptr(offs) if offsets else None))
+    if offsets:
+        return mosaic, (left, top, right, bottom), offs
+    return mosaic, (left, top, right, bottom)
+
+
+def merge_subswath(ctx, src, dst, top, left, out_float=False):
+    """dst[top:, left:] takes src where src has data (interfere.py:170-179)."""
+    dt = torch.float32 if out_float else torch.complex64
+    _need(src, dt, None, "subswath")
+    _need(dst, dt, None, "image")
+    h, w = src.shape
+    if top < 0 or left < 0 or top + h > dst.shape[0] or left + w > dst.shape[1]:
+        raise ValueError(f"subswath {h} x {w} at ({top}, {left}) outside {tuple(dst.shape)}")
+    if not dst.is_contiguous():
+        raise ValueError("image must be contiguous")
+    base = N.ptr(dst).value + (top * dst.shape[1] + left) * dst.element_size()
+    N.check(ctx.lib.s1ifg_merge_subswath(ctx.handle, N.ptr(src), h, w, int(bool(out_float)),
+                                         ctypes.c_void_p(base), dst.shape[1]))
+    return dst
+
+
 def goldstein(ctx, ifg, alpha=0.5, n_win=32, mode="fp32", out=None):
     """Goldstein filter (goldstein.cu:87-408), deterministic overlap-add."""
     _need(ifg, torch.complex64, None, "ifg")

@@ -129,7 +129,7 @@ def test_stitch_subswath_recovers_offsets(tmp_path):
         bursts.append(((nrow0, ncol0, 0, t, ncol0, b), data))
         w = name.split("_")
         ids.append((w[1] + w[2], w[7] + w[8]))
-    want, _ = OI.stitch_subswath(bursts, ids)
+    want, _, _ = OI.stitch_subswath(bursts, ids)
     got = IF.stitch_subswath(files, False)
     assert np.allclose(got.data, want, atol=1e-6)
     assert np.allclose(np.angle(got.data * np.conj(base)), 0, atol=1e-5)
