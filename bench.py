@@ -441,9 +441,9 @@ def run_stitch(args):
     ms = start.elapsed_time(stop) / steps
     P, M, O = h * w, mos.numel(), ov * w
     # compulsory traffic: burst 0 read and the mosaic written once; per later
-    # burst its pixels and the mosaic window read, its new pixels written,
-    # and the overlap read once more for the offset estimate
-    alg = 8 * (P + M + (n - 1) * (3 * P + O))
+    # burst its pixels read and its new pixels (all but the overlap) written,
+    # and the overlap's mosaic and burst pixels read for the offset estimate
+    alg = 8 * (P + M + (n - 1) * (2 * P + O))
     gbs = alg / (ms * 1e-3) / 1e9
     r = dict(workload=f"{n} bursts of {h} x {w} complex64 look cells, {ov}-row overlaps, "
                       f"every boundary estimated",

@@ -4,25 +4,28 @@
 // The bursts of a subswath are processed in order on the context stream; the
 // mosaic is the only state between them.  Per burst i >= 1 whose ids differ
 // from burst i-1's:
-//   1. k_ov_partial: over the burst's window, pixels where both the mosaic
+//   1. k_ov_partial: over the burst rows that earlier bursts' boxes cover
+//      (elsewhere the mosaic holds no data), pixels where both the mosaic
 //      and the burst hold data (the overlap mask) add their phase difference
 //      d = conj(old) * new (complex) or exp(i (new - old)) (phase images) to
-//      FP64 block partials, with the overlap count and its row range;
-//   2. k_ov_finalize (one block): partials summed in block order, m = angle
-//      of the mean (float32, as np.angle of the complex64 mean), the ranks
-//      (n-1)/2 and n/2 of the median, histograms cleared;
-//   3. three passes of k_ov_hist + k_ov_select: radix selection (11, 11 and
-//      10 bits) of both ranks among the float32 keys angle(d * exp(-i m)),
-//      over the overlap rows only; the last pass sets the running offset
-//      m + median (float32 arithmetic: np.median of float32 is the middle
-//      value or the float32 mean of the two middle values);
+//      FP64 block partials, with the overlap count and its row range.  The
+//      last block to finish sums the partials in block order: m = angle of
+//      the mean (float32, as np.angle of the complex64 mean), the ranks
+//      (n-1)/2 and n/2 of the median;
+//   2. three passes of k_ov_hist: radix selection (11, 11 and 10 bits) of
+//      both ranks among the float32 keys angle(d * exp(-i m)) over the
+//      overlap rows; the last block of each pass picks the digits, and that
+//      of the last pass sets the running offset m + median (float32
+//      arithmetic: np.median of float32 is the middle value or the float32
+//      mean of the two middle values);
 // then, for every burst i >= 1, k_apply replaces mosaic pixels holding no
 // data by the burst's pixels rotated (or shifted and wrapped) by the running
 // offset, which carries over to bursts whose ids match their predecessor's,
 // or whose overlap is empty, as in the reference.
 //
-// Everything stays on the device (no host round trip per burst); the
-// partials are summed in a fixed order, so results do not depend on timing.
+// Everything stays on the device (no host round trip per burst).  Partials
+// are summed in a fixed order and histograms are integer counts, so results
+// do not depend on timing.
 #include <climits>
 
 #include "s1ifg_internal.h"
@@ -34,6 +37,9 @@ namespace {
 constexpr int kOvBlocks = 1024;    // fixed grid of the partial sums
 constexpr int kBins = 2048;        // radix digit of the first two passes (11 bits)
 constexpr float kPiF = 3.14159265358979f;
+
+constexpr int kHistBlocks = 256;   // grid of the histogram passes
+constexpr int kChunk = 1024;       // columns per work item (4 per thread)
 
 struct OvPartial {
   double re, im;
@@ -60,6 +66,7 @@ struct Win {
   int h, w;          // burst window rows, columns
   int r0, c0;        // window origin in the mosaic
   long ld;           // mosaic row stride (elements)
+  int o0, o1;        // burst rows [o0, o1) inside earlier bursts' boxes
 };
 
 __device__ __forceinline__ bool has_data(float2 v) { return v.x != 0.f; }
@@ -93,6 +100,13 @@ __device__ __forceinline__ float key_value(unsigned u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
+// rotation exp(-i m) in complex64 (cos m, -sin m)
+__device__ __forceinline__ float2 unrotate(float m) {
+  float s, c;
+  sincosf(m, &s, &c);
+  return make_float2(c, -s);
+}
+
 template <typename T>
 __device__ __forceinline__ float median_key(T o, T n, float2 rot) {
   const float2 d = cmul(phase_diff(o, n), rot);
@@ -115,21 +129,57 @@ __global__ __launch_bounds__(256) void k_place_first(const T *__restrict__ b, in
   }
 }
 
+// Fixed-order block sums: s_re / s_im / s_n added, s_lo / s_hi min / max.
+struct BlockSums {
+  double re[256], im[256];
+  long long n[256];
+  int lo[256], hi[256];
+};
+
+__device__ __forceinline__ void block_sums(BlockSums &S) {
+  const int t = threadIdx.x;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      S.re[t] += S.re[t + h];
+      S.im[t] += S.im[t + h];
+      S.n[t] += S.n[t + h];
+      S.lo[t] = min(S.lo[t], S.lo[t + h]);
+      S.hi[t] = max(S.hi[t], S.hi[t + h]);
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_ov_partial(const T *__restrict__ b, Win w,
                                                     const T *__restrict__ mosaic,
                                                     Workspace *__restrict__ ws) {
+  __shared__ BlockSums S;
+  const int t = threadIdx.x;
   double sr = 0.0, si = 0.0;
   long long n = 0;
   int rmin = INT_MAX, rmax = -1;
-  for (int r = blockIdx.x; r < w.h; r += kOvBlocks) {
+  const int nchunk = (w.w + kChunk - 1) / kChunk;
+  const long items = (long)(w.o1 - w.o0) * nchunk;
+  for (long it = blockIdx.x; it < items; it += kOvBlocks) {
+    const int rr = (int)(it / nchunk);
+    const int r = w.o0 + rr;
+    const int cb = (int)(it - (long)rr * nchunk) * kChunk, ce = min(cb + kChunk, w.w);
     const T *brow = b + (long)r * w.w;
     const T *mrow = mosaic + (long)(w.r0 + r) * w.ld + w.c0;
     bool any = false;
-    for (int c = threadIdx.x; c < w.w; c += 256) {
-      const T o = mrow[c], v = brow[c];
-      if (has_data(o) && nonzero(v)) {
-        const float2 d = phase_diff(o, v);
+    T o[4], v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int c = cb + t + 256 * k;
+      o[k] = c < ce ? mrow[c] : T{};
+      v[k] = c < ce ? brow[c] : T{};
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (has_data(o[k]) && nonzero(v[k])) {
+        const float2 d = phase_diff(o[k], v[k]);
         sr += (double)d.x;
         si += (double)d.y;
         ++n;
@@ -141,116 +191,105 @@ __global__ __launch_bounds__(256) void k_ov_partial(const T *__restrict__ b, Win
       rmax = max(rmax, r);
     }
   }
-  __shared__ double s_re[256], s_im[256];
-  __shared__ long long s_n[256];
-  __shared__ int s_lo[256], s_hi[256];
-  const int t = threadIdx.x;
-  s_re[t] = sr;
-  s_im[t] = si;
-  s_n[t] = n;
-  s_lo[t] = rmin;
-  s_hi[t] = rmax;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (t < h) {
-      s_re[t] += s_re[t + h];
-      s_im[t] += s_im[t + h];
-      s_n[t] += s_n[t + h];
-      s_lo[t] = min(s_lo[t], s_lo[t + h]);
-      s_hi[t] = max(s_hi[t], s_hi[t + h]);
-    }
-    __syncthreads();
-  }
-  if (t == 0) ws->part[blockIdx.x] = OvPartial{s_re[0], s_im[0], s_n[0], s_lo[0], s_hi[0]};
+  S.re[t] = sr;
+  S.im[t] = si;
+  S.n[t] = n;
+  S.lo[t] = rmin;
+  S.hi[t] = rmax;
+  block_sums(S);
+  if (t == 0) ws->part[blockIdx.x] = OvPartial{S.re[0], S.im[0], S.n[0], S.lo[0], S.hi[0]};
 }
 
-// One block of 256 threads: the block partials in a fixed order.
+// One block: the partials in block order (4 per thread, then the tree).
 __global__ __launch_bounds__(256) void k_ov_finalize(Workspace *__restrict__ ws) {
+  __shared__ BlockSums S;
   const int t = threadIdx.x;
+  double sr, si;
+  long long n;
+  int rmin, rmax;
   constexpr int kPer = kOvBlocks / 256;
-  double sr = 0.0, si = 0.0;
-  long long n = 0;
-  int lo = INT_MAX, hi = -1;
+  sr = 0.0;
+  si = 0.0;
+  n = 0;
+  rmin = INT_MAX;
+  rmax = -1;
   for (int k = 0; k < kPer; ++k) {
     const OvPartial p = ws->part[t * kPer + k];
     sr += p.re;
     si += p.im;
     n += p.n;
-    lo = min(lo, p.rmin);
-    hi = max(hi, p.rmax);
+    rmin = min(rmin, p.rmin);
+    rmax = max(rmax, p.rmax);
   }
-  __shared__ double s_re[256], s_im[256];
-  __shared__ long long s_n[256];
-  __shared__ int s_lo[256], s_hi[256];
-  s_re[t] = sr;
-  s_im[t] = si;
-  s_n[t] = n;
-  s_lo[t] = lo;
-  s_hi[t] = hi;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (t < h) {
-      s_re[t] += s_re[t + h];
-      s_im[t] += s_im[t + h];
-      s_n[t] += s_n[t + h];
-      s_lo[t] = min(s_lo[t], s_lo[t + h]);
-      s_hi[t] = max(s_hi[t], s_hi[t + h]);
-    }
-    __syncthreads();
-  }
-  for (int k = t; k < 2 * kBins; k += 256) (&ws->hist[0][0])[k] = 0u;
+  S.re[t] = sr;
+  S.im[t] = si;
+  S.n[t] = n;
+  S.lo[t] = rmin;
+  S.hi[t] = rmax;
+  block_sums(S);
   if (t == 0) {
     StitchState &st = ws->st;
-    const long long cnt = s_n[0];
+    const long long cnt = S.n[0];
     st.n = cnt;
     if (cnt > 0) {
       // np.mean of the complex64 differences, then np.angle in float32
-      const float mr = (float)(s_re[0] / (double)cnt), mi = (float)(s_im[0] / (double)cnt);
+      const float mr = (float)(S.re[0] / (double)cnt), mi = (float)(S.im[0] / (double)cnt);
       st.m = atan2f(mi, mr);
       st.k[0] = (cnt - 1) / 2;
       st.k[1] = cnt / 2;
       st.prefix[0] = st.prefix[1] = 0u;
-      st.rmin = s_lo[0];
-      st.rmax = s_hi[0];
+      st.rmin = S.lo[0];
+      st.rmax = S.hi[0];
     }
   }
 }
 
-// rotation exp(-i m) in complex64 (cos m, -sin m)
-__device__ __forceinline__ float2 unrotate(float m) {
-  float s, c;
-  sincosf(m, &s, &c);
-  return make_float2(c, -s);
-}
-
-// Histogram of one radix digit (pass p: bits [shift, shift + width)) of the
-// keys whose higher bits equal each target's selected prefix; overlap rows
-// only.
+// Radix digit pass over the overlap rows: histogram of bits [shift, shift +
+// width) of the keys whose higher bits equal each target's selected prefix;
+// the last block picks, for each target, the digit whose cumulative count
+// passes the remaining rank, and clears the histogram.  After the last pass
+// (shift 0) the running offset becomes m + median.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ov_hist(const T *__restrict__ b, Win w,
                                                  const T *__restrict__ mosaic,
                                                  Workspace *__restrict__ ws, int shift, int width) {
   __shared__ unsigned h[2][kBins];
-  const StitchState st = ws->st;
-  if (st.n == 0) return;
+  StitchState &gst = ws->st;
+  const long long cnt = gst.n;
+  if (cnt == 0) return;
   const int t = threadIdx.x;
   const int nb = 1 << width;
+  const unsigned p0 = gst.prefix[0], p1 = gst.prefix[1];
+  const int rmin = gst.rmin, rmax = gst.rmax;
+  const float m = gst.m;
   for (int k = t; k < 2 * kBins; k += 256) (&h[0][0])[k] = 0u;
   __syncthreads();
   const unsigned hmask = (shift + width >= 32) ? 0u : (0xffffffffu << (shift + width));
   const unsigned dmask = (unsigned)nb - 1u;
-  const float2 rot = unrotate(st.m);
-  const bool same = st.prefix[0] == st.prefix[1];
-  for (int r = st.rmin + blockIdx.x; r <= st.rmax; r += gridDim.x) {
+  const float2 rot = unrotate(m);
+  const bool same = p0 == p1;
+  const int nchunk = (w.w + kChunk - 1) / kChunk;
+  const long items = (long)(rmax - rmin + 1) * nchunk;
+  for (long it = blockIdx.x; it < items; it += kHistBlocks) {
+    const int rr = (int)(it / nchunk);
+    const int r = rmin + rr;
+    const int cb = (int)(it - (long)rr * nchunk) * kChunk, ce = min(cb + kChunk, w.w);
     const T *brow = b + (long)r * w.w;
     const T *mrow = mosaic + (long)(w.r0 + r) * w.ld + w.c0;
-    for (int c = t; c < w.w; c += 256) {
-      const T o = mrow[c], v = brow[c];
-      if (!(has_data(o) && nonzero(v))) continue;
-      const unsigned u = order_key(median_key(o, v, rot));
+    T o[4], v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int c = cb + t + 256 * k;
+      o[k] = c < ce ? mrow[c] : T{};
+      v[k] = c < ce ? brow[c] : T{};
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (!(has_data(o[k]) && nonzero(v[k]))) continue;
+      const unsigned u = order_key(median_key(o[k], v[k], rot));
       const unsigned d = (u >> shift) & dmask;
-      if ((u & hmask) == st.prefix[0]) atomicAdd(&h[0][d], 1u);
-      if (!same && (u & hmask) == st.prefix[1]) atomicAdd(&h[1][d], 1u);
+      if ((u & hmask) == p0) atomicAdd(&h[0][d], 1u);
+      if (!same && (u & hmask) == p1) atomicAdd(&h[1][d], 1u);
     }
   }
   __syncthreads();
@@ -265,32 +304,32 @@ __global__ __launch_bounds__(256) void k_ov_hist(const T *__restrict__ b, Win w,
 // pass (shift 0) the running offset becomes m + median.
 __global__ __launch_bounds__(256) void k_ov_select(Workspace *__restrict__ ws, int shift,
                                                    int width) {
-  StitchState &st = ws->st;
-  if (st.n == 0) return;
-  const int t = threadIdx.x;
-  const int nb = 1 << width;
-  const int per = nb / 256;
-  const bool same = st.prefix[0] == st.prefix[1];
   __shared__ unsigned long long s_c[256];
   __shared__ unsigned s_pick[2];
   __shared__ long long s_rest[2];
+  StitchState &gst = ws->st;
+  const long long cnt = gst.n;
+  if (cnt == 0) return;
+  const int t = threadIdx.x;
+  const int nb = 1 << width;
+  const bool same = gst.prefix[0] == gst.prefix[1];
+  const float m = gst.m;
+  const int per = nb / 256;
   for (int tg = 0; tg < 2; ++tg) {
     const unsigned *hs = ws->hist[same ? 0 : tg];
     unsigned long long c = 0;
     for (int k = 0; k < per; ++k) c += hs[t * per + k];
+    // inclusive scan of the per-thread counts (Hillis-Steele)
     s_c[t] = c;
     __syncthreads();
-    if (t == 0) {
-      unsigned long long run = 0;
-      for (int k = 0; k < 256; ++k) {
-        const unsigned long long x = s_c[k];
-        s_c[k] = run;
-        run += x;
-      }
+    for (int o = 1; o < 256; o <<= 1) {
+      const unsigned long long x = t >= o ? s_c[t - o] : 0ull;
+      __syncthreads();
+      s_c[t] += x;
+      __syncthreads();
     }
-    __syncthreads();
-    const long long want = st.k[tg];
-    const unsigned long long lo = s_c[t], hi = lo + c;
+    const long long want = gst.k[tg];
+    const unsigned long long hi = s_c[t], lo = hi - c;
     if ((unsigned long long)want >= lo && (unsigned long long)want < hi) {
       unsigned long long run = lo;
       for (int k = 0; k < per; ++k) {
@@ -308,13 +347,13 @@ __global__ __launch_bounds__(256) void k_ov_select(Workspace *__restrict__ ws, i
   for (int k = t; k < 2 * kBins; k += 256) (&ws->hist[0][0])[k] = 0u;
   if (t == 0) {
     for (int tg = 0; tg < 2; ++tg) {
-      st.prefix[tg] |= s_pick[tg] << shift;
-      st.k[tg] = s_rest[tg];
+      gst.prefix[tg] |= s_pick[tg] << shift;
+      gst.k[tg] = s_rest[tg];
     }
     if (shift == 0) {
-      const float a = key_value(st.prefix[0]), b = key_value(st.prefix[1]);
-      const float med = (st.n & 1) ? a : __fmul_rn(__fadd_rn(a, b), 0.5f);
-      st.off = __fadd_rn(st.m, med);
+      const float a = key_value(gst.prefix[0]), bb = key_value(gst.prefix[1]);
+      const float med = (cnt & 1) ? a : __fmul_rn(__fadd_rn(a, bb), 0.5f);
+      gst.off = __fadd_rn(m, med);
     }
   }
 }
@@ -322,7 +361,8 @@ __global__ __launch_bounds__(256) void k_ov_select(Workspace *__restrict__ ws, i
 // Replace the mosaic pixels without data by the burst's (interfere.py:
 // 135-143): rotated by exp(-i off) (complex64), or shifted by -off and
 // wrapped once into [-pi, pi] (float32) when off != 0.  One burst row per
-// block.  Block 0 records the offset used.
+// block; rows outside earlier bursts' boxes hold no data in the mosaic, so
+// there the mosaic is not read.  Block 0 records the offset used.
 template <typename T>
 __device__ __forceinline__ T corrected(T v, float off);
 template <>
@@ -346,9 +386,24 @@ __global__ __launch_bounds__(256) void k_apply(const T *__restrict__ b, Win w,
   const int r = blockIdx.x;
   const T *brow = b + (long)r * w.w;
   T *mrow = mosaic + (long)(w.r0 + r) * w.ld + w.c0;
-  for (int c = threadIdx.x; c < w.w; c += 256) {
-    const T o = mrow[c], v = brow[c];
-    if (!has_data(o) && has_data(v)) mrow[c] = off != 0.f ? corrected(v, off) : v;
+  const bool covered = r >= w.o0 && r < w.o1;
+  constexpr int kU = 8;  // loads in flight per thread
+  for (int c0 = threadIdx.x; c0 < w.w; c0 += 256 * kU) {
+    T v[kU];
+    bool put[kU];
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int c = c0 + 256 * k;
+      v[k] = c < w.w ? brow[c] : T{};
+    }
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int c = c0 + 256 * k;
+      put[k] = has_data(v[k]) && !(covered && has_data(mrow[c]));
+    }
+#pragma unroll
+    for (int k = 0; k < kU; ++k)
+      if (put[k]) mrow[c0 + 256 * k] = off != 0.f ? corrected(v[k], off) : v[k];
   }
   if (offsets && r == 0 && threadIdx.x == 0) offsets[burst] = (double)off;
 }
@@ -369,6 +424,7 @@ __global__ void k_init_state(Workspace *__restrict__ ws, double *__restrict__ of
     ws->st.n = 0;
     if (offsets) offsets[0] = 0.0;
   }
+  for (int k = threadIdx.x; k < 2 * kBins; k += blockDim.x) (&ws->hist[0][0])[k] = 0u;
 }
 
 template <typename T>
@@ -380,17 +436,23 @@ int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosa
   if (rc) return rc;
   Workspace *ws = (Workspace *)p;
   hipStream_t s = ctx->stream;
-  hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, s, ws, offsets);
+  hipLaunchKernelGGL(k_init_state, dim3(1), dim3(256), 0, s, ws, offsets);
   S1_LAUNCH_CHECK("k_init_state");
   const s1ifg_stitch_burst &b0 = bursts[0];
   hipLaunchKernelGGL(k_place_first<T>, dim3(mh), dim3(256), 0, s, (const T *)b0.data,
                      b0.bottom - b0.top, b0.right - b0.left, b0.top - top, b0.left - left, mosaic,
                      mh, mw);
   S1_LAUNCH_CHECK("k_place_first");
+  // rows of the mosaic that earlier bursts may have written (burst 0's box
+  // clipped to the bounds, then the later bursts' boxes)
+  int cov0 = max(b0.top, top), cov1 = min(b0.bottom, bottom);
   for (int i = 1; i < n; ++i) {
     const s1ifg_stitch_burst &bi = bursts[i];
     const T *bd = (const T *)bi.data;
-    Win w{bi.bottom - bi.top, bi.right - bi.left, bi.top - top, bi.left - left, (long)mw};
+    const int h = bi.bottom - bi.top;
+    Win w{h, bi.right - bi.left, bi.top - top, bi.left - left, (long)mw, 0, 0};
+    w.o0 = min(max(cov0 - bi.top, 0), h);
+    w.o1 = max(min(cov1 - bi.top, h), w.o0);
     if (bi.estimate) {
       hipLaunchKernelGGL(k_ov_partial<T>, dim3(kOvBlocks), dim3(256), 0, s, bd, w,
                          (const T *)mosaic, ws);
@@ -399,7 +461,7 @@ int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosa
       S1_LAUNCH_CHECK("k_ov_finalize");
       const int shifts[3] = {21, 10, 0}, widths[3] = {11, 11, 10};
       for (int pss = 0; pss < 3; ++pss) {
-        hipLaunchKernelGGL(k_ov_hist<T>, dim3(min(w.h, 512)), dim3(256), 0, s, bd, w,
+        hipLaunchKernelGGL(k_ov_hist<T>, dim3(kHistBlocks), dim3(256), 0, s, bd, w,
                            (const T *)mosaic, ws, shifts[pss], widths[pss]);
         S1_LAUNCH_CHECK("k_ov_hist");
         hipLaunchKernelGGL(k_ov_select, dim3(1), dim3(256), 0, s, ws, shifts[pss],
@@ -407,9 +469,11 @@ int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosa
         S1_LAUNCH_CHECK("k_ov_select");
       }
     }
-    hipLaunchKernelGGL(k_apply<T>, dim3(w.h), dim3(256), 0, s, bd, w, mosaic,
+    hipLaunchKernelGGL(k_apply<T>, dim3(h), dim3(256), 0, s, bd, w, mosaic,
                        (const Workspace *)ws, offsets, i);
     S1_LAUNCH_CHECK("k_apply");
+    cov0 = min(cov0, bi.top);
+    cov1 = max(cov1, bi.bottom);
   }
   return S1IFG_OK;
 }

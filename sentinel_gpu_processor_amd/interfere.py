@@ -5,8 +5,8 @@ The per-burst crossmul + looks runs in-process on the GPU through
 ``daemon.CrossmulDriver`` (the reference launches its crossmul_daemon
 executable and parses its stdout, interfere.py:337-533); the file contract
 (.gslc in, per-burst .int out, OK / FAIL accounting, skip-existing resume)
-is unchanged.  Stitching keeps the reference's host algorithm
-(interfere.py:81-180).
+is unchanged.  Stitching (interfere.py:81-180) runs on the GPU
+(s1ifg_stitch_subswath, s1ifg_merge_subswath).
 """
 
 import glob
@@ -17,6 +17,8 @@ from typing import List, Optional, Tuple
 import numpy as np
 
 from . import geocoordinates
+from . import kernels as K
+from .runtime import context, device
 from .sario import BurstGroup, CroppedImage, Subswath
 
 try:
@@ -56,49 +58,29 @@ def interfere_subswath(ref_subswath, sec_subswath, ifg_path) -> List[Tuple[str, 
 
 def stitch_subswath(burst_ifgs, out_float):
     """Mosaic the burst interferograms of one subswath, removing the phase
-    offset at scene boundaries (interfere.py:81-150)."""
+    offset at scene boundaries (interfere.py:81-150).  The bursts are read
+    on the host; the mosaic (overlap mean + median offsets, replacement) is
+    built on the GPU by s1ifg_stitch_subswath.  The burst files are removed
+    afterwards, as in the reference."""
+    import torch  # noqa: F401  (device tensors)
     subswath = Subswath(burst_ifgs)
     left, top, right, bottom = subswath.bounds()
     dtype = np.float32 if out_float else np.complex64
-    mean_phase_diff = 0.0
-    prev1 = prev2 = None
-    ifg_data = None
-    for i, burst in enumerate(subswath.bursts):
+    dev = device()
+    bursts, prev = [], None
+    for burst in subswath.bursts:
         burst.dtype = dtype
         words = os.path.basename(burst.data).split("_")
-        uid1 = words[1] + words[2]
-        uid2 = words[7] + words[8]
-        if i == 0:
-            ifg_data = burst.load_data(left, top, right, bottom)
-            prev1, prev2 = uid1, uid2
-            continue
-        old = ifg_data[burst.top - top:burst.bottom - top, burst.left - left:burst.right - left]
-        new = burst.load_data(burst.left, burst.top, burst.right, burst.bottom)
-        if out_float:
-            replace = (old == 0) & (new != 0)
-        else:
-            replace = (old.real == 0) & (new.real != 0)
-        ovm = None
-        if uid1 != prev1 or uid2 != prev2:
-            ovm = (old != 0) & (new != 0) if out_float else (old.real != 0) & (new != 0)
-        prev1, prev2 = uid1, uid2
-        if ovm is not None and np.any(ovm):
-            if out_float:
-                pd_ = np.exp(1j * (-old[ovm] + new[ovm]))
-            else:
-                pd_ = np.conj(old[ovm]) * new[ovm]
-            mean_phase_diff = np.angle(np.mean(pd_))
-            pd_ = pd_ * np.exp(-1j * mean_phase_diff)
-            mean_phase_diff += np.median(np.angle(pd_))
-        if mean_phase_diff != 0:
-            if out_float:
-                new = new - mean_phase_diff
-                new[new > np.pi] -= 2 * np.pi
-                new[new < -np.pi] += 2 * np.pi
-            else:
-                new = new * np.exp(-1j * mean_phase_diff)
-        old[replace] = new[replace]
-    ifg = CroppedImage(subswath.nrow0, subswath.ncol0, left, top, right, bottom, ifg_data)
+        uid = (words[1] + words[2], words[7] + words[8])
+        data = burst.load_data(burst.left, burst.top, burst.right, burst.bottom)
+        bursts.append((burst.left, burst.top, burst.right, burst.bottom,
+                       K.from_numpy(np.ascontiguousarray(data, dtype=dtype), dev),
+                       prev is not None and uid != prev))
+        prev = uid
+    mos, _ = K.stitch_subswath(context(), bursts, out_float=out_float,
+                               bounds=(left, top, right, bottom))
+    ifg = CroppedImage(subswath.nrow0, subswath.ncol0, left, top, right, bottom,
+                       K.to_numpy(mos), dtype=dtype)
     for f in burst_ifgs:
         os.remove(f)
     return ifg
@@ -106,7 +88,9 @@ def stitch_subswath(burst_ifgs, out_float):
 
 def stitch(burst_pairs, outfile, out_float):
     """Mosaic the three subswaths into one raw (headerless) image
-    (interfere.py:153-180)."""
+    (interfere.py:153-180); the merge runs on the GPU
+    (s1ifg_merge_subswath)."""
+    import torch
     subs = []
     for i in range(1, 4):
         bp = [b for b in burst_pairs if (f"iw{i}" in b) and os.path.exists(b)]
@@ -117,12 +101,14 @@ def stitch(burst_pairs, outfile, out_float):
         return
     nrow0, ncol0 = subs[0].nrow0, subs[0].ncol0
     dtype = np.float32 if out_float else np.complex64
-    mm = np.memmap(outfile, dtype=dtype, mode="w+", shape=(nrow0, ncol0))
+    ctx = context()
+    full = torch.zeros((nrow0, ncol0), dtype=torch.float32 if out_float else torch.complex64,
+                       device=device())
     for s in subs:
-        old = mm[s.top:s.bottom, s.left:s.right]
-        new = s.data
-        rep = (new != 0) if out_float else (new.real != 0)
-        old[rep] = new[rep]
+        K.merge_subswath(ctx, K.from_numpy(np.ascontiguousarray(s.data, dtype=dtype), device()),
+                         full, s.top, s.left, out_float)
+    mm = np.memmap(outfile, dtype=dtype, mode="w+", shape=(nrow0, ncol0))
+    mm[:] = K.to_numpy(full)
     mm.flush()
 
 

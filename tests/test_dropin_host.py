@@ -109,6 +109,7 @@ def test_reorder_tasks_for_cache_locality():
     assert all(o in t for o in out)
 
 
+@pytest.mark.gpu
 def test_stitch_subswath_recovers_offsets(tmp_path):
     """Bursts from two scenes with a known phase offset across the scene
     boundary: the stitched mosaic is continuous (interfere.py:102-143), and

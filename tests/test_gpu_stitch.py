@@ -141,3 +141,43 @@ def test_stitch_rejects_burst_outside_bounds():
     with pytest.raises(Exception, match="outside"):
         K.stitch_subswath(context(), [(0, 0, 10, 10, a, True), (0, 5, 12, 15, b, True),
                                       (0, 10, 10, 20, c, True)])
+
+
+@pytest.mark.parametrize("out_float", [False, True])
+def test_dropin_stitch_files(tmp_path, out_float):
+    """interfere.stitch on per-burst .int files of two subswaths (the
+    reference's file contract, interfere.py:153-180) against the oracle's
+    stitch + merge; the burst files are removed."""
+    import os
+
+    from sentinel_gpu_processor_amd import interfere as IF
+    from sentinel_gpu_processor_amd import sario as SA
+    rng = np.random.default_rng(15)
+    nrow0, ncol0 = 90, 120
+    dt = np.float32 if out_float else np.complex64
+    files, subs = [], []
+    for sw, (l, r) in ((1, (0, 70)), (2, (60, 120))):
+        spans = [(0, 35), (30, 62), (57, 90)]
+        bursts, ids = [], []
+        for k, (t, b) in enumerate(spans):
+            ph = rng.uniform(-3, 3, (b - t, r - l)) + 0.4 * k
+            data = (np.angle(np.exp(1j * ph)) if out_float else np.exp(1j * ph)).astype(dt)
+            data[rng.random(data.shape) < 0.05] = 0
+            name = f"x_d1_s{k}_a_b_c_d_d2_s{k}_iw{sw}.int"
+            f = str(tmp_path / name)
+            SA.save_cropped(f, data, nrow0, ncol0, l, t, r, b)
+            files.append(f)
+            bursts.append(((nrow0, ncol0, l, t, r, b), data))
+            w = name.split("_")
+            ids.append((w[1] + w[2], w[7] + w[8]))
+        mos, bnd, _ = OI.stitch_subswath(bursts, ids, out_float=out_float)
+        subs.append((bnd, mos))
+    want = OI.merge_subswaths(subs, nrow0, ncol0, out_float)
+    out = str(tmp_path / "pair.int")
+    IF.stitch(files, out, out_float)
+    got = np.fromfile(out, dtype=dt).reshape(nrow0, ncol0)
+    if out_float:
+        assert np.all(np.abs(np.angle(np.exp(1j * (got.astype(np.float64) - want)))) < TOL)
+    else:
+        assert np.allclose(got, want, atol=4 * TOL, rtol=0)
+    assert not any(os.path.exists(f) for f in files)
