@@ -8,16 +8,16 @@
 //      (elsewhere the mosaic holds no data), pixels where both the mosaic
 //      and the burst hold data (the overlap mask) add their phase difference
 //      d = conj(old) * new (complex) or exp(i (new - old)) (phase images) to
-//      FP64 block partials, with the overlap count and its row range.  The
-//      last block to finish sums the partials in block order: m = angle of
-//      the mean (float32, as np.angle of the complex64 mean), the ranks
-//      (n-1)/2 and n/2 of the median;
+//      FP64 block partials, with the overlap count and its row range;
 //   2. three passes of k_ov_hist: radix selection (11, 11 and 10 bits) of
-//      both ranks among the float32 keys angle(d * exp(-i m)) over the
-//      overlap rows; the last block of each pass picks the digits, and that
-//      of the last pass sets the running offset m + median (float32
-//      arithmetic: np.median of float32 is the middle value or the float32
-//      mean of the two middle values);
+//      the ranks (n-1)/2 and n/2 among the float32 keys angle(d exp(-i m)),
+//      m the angle of the mean difference, over the overlap rows.  Each
+//      block derives the state it filters with itself (pass 0 sums the
+//      partials in block order, pass p > 0 selects the digits of pass p-1's
+//      histogram), so no one-block kernel sits between the passes;
+//   3. k_ov_median (one block): the last digits; the running offset becomes
+//      m + median (float32 arithmetic: np.median of float32 is the middle
+//      value or the float32 mean of the two middle values);
 // then, for every burst i >= 1, k_apply replaces mosaic pixels holding no
 // data by the burst's pixels rotated (or shifted and wrapped) by the running
 // offset, which carries over to bursts whose ids match their predecessor's,
@@ -47,18 +47,24 @@ struct OvPartial {
   int rmin, rmax;
 };
 
+// Selection state of one radix pass: the key bits selected so far and the
+// remaining rank, for the (n-1)/2 and n/2 order statistics.
+struct Sel {
+  unsigned prefix[2];
+  long long k[2];
+};
+
 struct StitchState {
   float off;            // running mean_phase_diff (0 before the first estimate)
   float m;              // angle of the mean phase difference
   long long n;          // overlap count of the current burst
-  long long k[2];       // remaining rank of the (n-1)/2 and n/2 order statistics
-  unsigned prefix[2];   // key bits selected so far
   int rmin, rmax;       // overlap rows (burst window rows)
+  Sel sel[3];           // sel[p]: the state histogram pass p filters with
 };
 
 struct Workspace {
   StitchState st;
-  unsigned hist[2][kBins];
+  unsigned hist[3][2][kBins];  // hist[p]: counts of pass p
   OvPartial part[kOvBlocks];
 };
 
@@ -200,19 +206,21 @@ __global__ __launch_bounds__(256) void k_ov_partial(const T *__restrict__ b, Win
   if (t == 0) ws->part[blockIdx.x] = OvPartial{S.re[0], S.im[0], S.n[0], S.lo[0], S.hi[0]};
 }
 
-// One block: the partials in block order (4 per thread, then the tree).
-__global__ __launch_bounds__(256) void k_ov_finalize(Workspace *__restrict__ ws) {
-  __shared__ BlockSums S;
-  const int t = threadIdx.x;
-  double sr, si;
+// The partials in block order (4 per thread, then the tree), by every
+// thread of a block: overlap count, angle m of the mean (float32, as
+// np.angle of the complex64 mean) and the overlap rows.
+struct Overlap {
   long long n;
+  float m;
   int rmin, rmax;
+};
+
+__device__ __forceinline__ Overlap sum_partials(const Workspace *ws, BlockSums &S) {
+  const int t = threadIdx.x;
   constexpr int kPer = kOvBlocks / 256;
-  sr = 0.0;
-  si = 0.0;
-  n = 0;
-  rmin = INT_MAX;
-  rmax = -1;
+  double sr = 0.0, si = 0.0;
+  long long n = 0;
+  int rmin = INT_MAX, rmax = -1;
   for (int k = 0; k < kPer; ++k) {
     const OvPartial p = ws->part[t * kPer + k];
     sr += p.re;
@@ -227,46 +235,135 @@ __global__ __launch_bounds__(256) void k_ov_finalize(Workspace *__restrict__ ws)
   S.lo[t] = rmin;
   S.hi[t] = rmax;
   block_sums(S);
-  if (t == 0) {
-    StitchState &st = ws->st;
-    const long long cnt = S.n[0];
-    st.n = cnt;
-    if (cnt > 0) {
-      // np.mean of the complex64 differences, then np.angle in float32
-      const float mr = (float)(S.re[0] / (double)cnt), mi = (float)(S.im[0] / (double)cnt);
-      st.m = atan2f(mi, mr);
-      st.k[0] = (cnt - 1) / 2;
-      st.k[1] = cnt / 2;
-      st.prefix[0] = st.prefix[1] = 0u;
-      st.rmin = S.lo[0];
-      st.rmax = S.hi[0];
-    }
+  Overlap o;
+  o.n = S.n[0];
+  o.m = 0.f;
+  if (o.n > 0) {
+    const float mr = (float)(S.re[0] / (double)o.n), mi = (float)(S.im[0] / (double)o.n);
+    o.m = atan2f(mi, mr);
   }
+  o.rmin = S.lo[0];
+  o.rmax = S.hi[0];
+  __syncthreads();  // S reusable
+  return o;
 }
 
-// Radix digit pass over the overlap rows: histogram of bits [shift, shift +
-// width) of the keys whose higher bits equal each target's selected prefix;
-// the last block picks, for each target, the digit whose cumulative count
-// passes the remaining rank, and clears the histogram.  After the last pass
-// (shift 0) the running offset becomes m + median.
+// One radix digit selection by every thread of a block: for each target, the
+// digit of histogram H (nb bins) whose cumulative count passes the remaining
+// rank.  Returns the next selection state (same value in every thread).
+struct SelScratch {
+  unsigned long long c[256];
+  unsigned pick[2];
+  long long rest[2];
+};
+
+__device__ __forceinline__ Sel select_digit(const unsigned (*H)[kBins], int nb, int shift,
+                                            const Sel &in, SelScratch &X) {
+  const int t = threadIdx.x;
+  const int per = nb / 256;
+  const bool same = in.prefix[0] == in.prefix[1];
+  for (int tg = 0; tg < 2; ++tg) {
+    const unsigned *hs = H[same ? 0 : tg];
+    unsigned x[8];
+    unsigned long long c = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      x[k] = k < per ? hs[t * per + k] : 0u;
+      c += x[k];
+    }
+    // inclusive scan of the per-thread counts (Hillis-Steele)
+    X.c[t] = c;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      const unsigned long long y = t >= o ? X.c[t - o] : 0ull;
+      __syncthreads();
+      X.c[t] += y;
+      __syncthreads();
+    }
+    const long long want = in.k[tg];
+    const unsigned long long hi = X.c[t], lo = hi - c;
+    if ((unsigned long long)want >= lo && (unsigned long long)want < hi) {
+      unsigned long long run = lo;
+      for (int k = 0; k < per; ++k) {
+        if ((unsigned long long)want < run + x[k]) {
+          X.pick[tg] = (unsigned)(t * per + k);
+          X.rest[tg] = want - (long long)run;
+          break;
+        }
+        run += x[k];
+      }
+    }
+    __syncthreads();
+  }
+  Sel out;
+  for (int tg = 0; tg < 2; ++tg) {
+    out.prefix[tg] = in.prefix[tg] | (X.pick[tg] << shift);
+    out.k[tg] = X.rest[tg];
+  }
+  __syncthreads();  // X reusable
+  return out;
+}
+
+constexpr int kShift[3] = {21, 10, 0}, kWidth[3] = {11, 11, 10};
+
+// Radix pass p over the overlap rows: histogram of bits [shift, shift +
+// width) of the keys angle(d exp(-i m)) whose higher bits equal each
+// target's selected prefix.  Every block first derives the selection state
+// it filters with, identically: pass 0 from the overlap partials, pass p > 0
+// by the digit selection over pass p-1's histogram; block 0 stores it (and
+// pass 0 the overlap figures) for the following kernels.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ov_hist(const T *__restrict__ b, Win w,
                                                  const T *__restrict__ mosaic,
-                                                 Workspace *__restrict__ ws, int shift, int width) {
+                                                 Workspace *__restrict__ ws, int pass) {
   __shared__ unsigned h[2][kBins];
+  __shared__ union {
+    BlockSums S;
+    SelScratch X;
+  } U;
   StitchState &gst = ws->st;
-  const long long cnt = gst.n;
-  if (cnt == 0) return;
   const int t = threadIdx.x;
+  long long cnt;
+  float m;
+  int rmin, rmax;
+  Sel sel;
+  if (pass == 0) {
+    const Overlap o = sum_partials(ws, U.S);
+    cnt = o.n;
+    m = o.m;
+    rmin = o.rmin;
+    rmax = o.rmax;
+    sel.prefix[0] = sel.prefix[1] = 0u;
+    sel.k[0] = (cnt - 1) / 2;
+    sel.k[1] = cnt / 2;
+    if (blockIdx.x == 0 && t == 0) {
+      gst.n = cnt;
+      if (cnt > 0) {
+        gst.m = m;
+        gst.rmin = rmin;
+        gst.rmax = rmax;
+        gst.sel[0] = sel;
+      }
+    }
+    if (cnt == 0) return;
+  } else {
+    cnt = gst.n;
+    if (cnt == 0) return;
+    m = gst.m;
+    rmin = gst.rmin;
+    rmax = gst.rmax;
+    sel = select_digit(ws->hist[pass - 1], 1 << kWidth[pass - 1], kShift[pass - 1],
+                       gst.sel[pass - 1], U.X);
+    if (blockIdx.x == 0 && t == 0) gst.sel[pass] = sel;
+  }
+  const int shift = kShift[pass], width = kWidth[pass];
   const int nb = 1 << width;
-  const unsigned p0 = gst.prefix[0], p1 = gst.prefix[1];
-  const int rmin = gst.rmin, rmax = gst.rmax;
-  const float m = gst.m;
   for (int k = t; k < 2 * kBins; k += 256) (&h[0][0])[k] = 0u;
   __syncthreads();
   const unsigned hmask = (shift + width >= 32) ? 0u : (0xffffffffu << (shift + width));
   const unsigned dmask = (unsigned)nb - 1u;
   const float2 rot = unrotate(m);
+  const unsigned p0 = sel.prefix[0], p1 = sel.prefix[1];
   const bool same = p0 == p1;
   const int nchunk = (w.w + kChunk - 1) / kChunk;
   const long items = (long)(rmax - rmin + 1) * nchunk;
@@ -293,68 +390,27 @@ __global__ __launch_bounds__(256) void k_ov_hist(const T *__restrict__ b, Win w,
     }
   }
   __syncthreads();
+  unsigned(*gh)[kBins] = ws->hist[pass];
   for (int k = t; k < nb; k += 256) {
-    if (h[0][k]) atomicAdd(&ws->hist[0][k], h[0][k]);
-    if (!same && h[1][k]) atomicAdd(&ws->hist[1][k], h[1][k]);
+    if (h[0][k]) atomicAdd(&gh[0][k], h[0][k]);
+    if (!same && h[1][k]) atomicAdd(&gh[1][k], h[1][k]);
   }
 }
 
-// One block: for each target, the digit whose cumulative count passes the
-// remaining rank; the histogram is cleared for the next pass.  After the last
-// pass (shift 0) the running offset becomes m + median.
-__global__ __launch_bounds__(256) void k_ov_select(Workspace *__restrict__ ws, int shift,
-                                                   int width) {
-  __shared__ unsigned long long s_c[256];
-  __shared__ unsigned s_pick[2];
-  __shared__ long long s_rest[2];
+// One block: the digit selection over the last pass's histogram gives both
+// order statistics; the running offset becomes m + median.  All histograms
+// are cleared for the next boundary.
+__global__ __launch_bounds__(256) void k_ov_median(Workspace *__restrict__ ws) {
+  __shared__ SelScratch X;
   StitchState &gst = ws->st;
   const long long cnt = gst.n;
   if (cnt == 0) return;
-  const int t = threadIdx.x;
-  const int nb = 1 << width;
-  const bool same = gst.prefix[0] == gst.prefix[1];
-  const float m = gst.m;
-  const int per = nb / 256;
-  for (int tg = 0; tg < 2; ++tg) {
-    const unsigned *hs = ws->hist[same ? 0 : tg];
-    unsigned long long c = 0;
-    for (int k = 0; k < per; ++k) c += hs[t * per + k];
-    // inclusive scan of the per-thread counts (Hillis-Steele)
-    s_c[t] = c;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-      const unsigned long long x = t >= o ? s_c[t - o] : 0ull;
-      __syncthreads();
-      s_c[t] += x;
-      __syncthreads();
-    }
-    const long long want = gst.k[tg];
-    const unsigned long long hi = s_c[t], lo = hi - c;
-    if ((unsigned long long)want >= lo && (unsigned long long)want < hi) {
-      unsigned long long run = lo;
-      for (int k = 0; k < per; ++k) {
-        const unsigned x = hs[t * per + k];
-        if ((unsigned long long)want < run + x) {
-          s_pick[tg] = (unsigned)(t * per + k);
-          s_rest[tg] = want - (long long)run;
-          break;
-        }
-        run += x;
-      }
-    }
-    __syncthreads();
-  }
-  for (int k = t; k < 2 * kBins; k += 256) (&ws->hist[0][0])[k] = 0u;
-  if (t == 0) {
-    for (int tg = 0; tg < 2; ++tg) {
-      gst.prefix[tg] |= s_pick[tg] << shift;
-      gst.k[tg] = s_rest[tg];
-    }
-    if (shift == 0) {
-      const float a = key_value(gst.prefix[0]), bb = key_value(gst.prefix[1]);
-      const float med = (cnt & 1) ? a : __fmul_rn(__fadd_rn(a, bb), 0.5f);
-      gst.off = __fadd_rn(m, med);
-    }
+  const Sel sel = select_digit(ws->hist[2], 1 << kWidth[2], kShift[2], gst.sel[2], X);
+  for (int k = threadIdx.x; k < 3 * 2 * kBins; k += 256) (&ws->hist[0][0][0])[k] = 0u;
+  if (threadIdx.x == 0) {
+    const float a = key_value(sel.prefix[0]), bb = key_value(sel.prefix[1]);
+    const float med = (cnt & 1) ? a : __fmul_rn(__fadd_rn(a, bb), 0.5f);
+    gst.off = __fadd_rn(gst.m, med);
   }
 }
 
@@ -424,7 +480,7 @@ __global__ void k_init_state(Workspace *__restrict__ ws, double *__restrict__ of
     ws->st.n = 0;
     if (offsets) offsets[0] = 0.0;
   }
-  for (int k = threadIdx.x; k < 2 * kBins; k += blockDim.x) (&ws->hist[0][0])[k] = 0u;
+  for (int k = threadIdx.x; k < 3 * 2 * kBins; k += blockDim.x) (&ws->hist[0][0][0])[k] = 0u;
 }
 
 template <typename T>
@@ -457,17 +513,13 @@ int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosa
       hipLaunchKernelGGL(k_ov_partial<T>, dim3(kOvBlocks), dim3(256), 0, s, bd, w,
                          (const T *)mosaic, ws);
       S1_LAUNCH_CHECK("k_ov_partial");
-      hipLaunchKernelGGL(k_ov_finalize, dim3(1), dim3(256), 0, s, ws);
-      S1_LAUNCH_CHECK("k_ov_finalize");
-      const int shifts[3] = {21, 10, 0}, widths[3] = {11, 11, 10};
       for (int pss = 0; pss < 3; ++pss) {
         hipLaunchKernelGGL(k_ov_hist<T>, dim3(kHistBlocks), dim3(256), 0, s, bd, w,
-                           (const T *)mosaic, ws, shifts[pss], widths[pss]);
+                           (const T *)mosaic, ws, pss);
         S1_LAUNCH_CHECK("k_ov_hist");
-        hipLaunchKernelGGL(k_ov_select, dim3(1), dim3(256), 0, s, ws, shifts[pss],
-                           widths[pss]);
-        S1_LAUNCH_CHECK("k_ov_select");
       }
+      hipLaunchKernelGGL(k_ov_median, dim3(1), dim3(256), 0, s, ws);
+      S1_LAUNCH_CHECK("k_ov_median");
     }
     hipLaunchKernelGGL(k_apply<T>, dim3(h), dim3(256), 0, s, bd, w, mosaic,
                        (const Workspace *)ws, offsets, i);
