@@ -14,7 +14,9 @@
 //      m the angle of the mean difference, over the overlap rows.  Each
 //      block derives the state it filters with itself (pass 0 sums the
 //      partials in block order, pass p > 0 selects the digits of pass p-1's
-//      histogram), so no one-block kernel sits between the passes;
+//      histogram), so no one-block kernel sits between the passes.  Pass
+//      0 stores the keys (one u32 per cell of the overlap rows) in the
+//      workspace; passes 1 and 2 read them instead of recomputing;
 //   3. k_ov_median (one block): the last digits; the running offset becomes
 //      m + median (float32 arithmetic: np.median of float32 is the middle
 //      value or the float32 mean of the two middle values);
@@ -26,6 +28,7 @@
 // Everything stays on the device (no host round trip per burst).  Partials
 // are summed in a fixed order and histograms are integer counts, so results
 // do not depend on timing.
+#include <algorithm>
 #include <climits>
 
 #include "s1ifg_internal.h"
@@ -315,7 +318,8 @@ constexpr int kShift[3] = {21, 10, 0}, kWidth[3] = {11, 11, 10};
 template <typename T>
 __global__ __launch_bounds__(256) void k_ov_hist(const T *__restrict__ b, Win w,
                                                  const T *__restrict__ mosaic,
-                                                 Workspace *__restrict__ ws, int pass) {
+                                                 Workspace *__restrict__ ws, int pass,
+                                                 unsigned *__restrict__ keys) {
   __shared__ unsigned h[2][kBins];
   __shared__ union {
     BlockSums S;
@@ -371,19 +375,37 @@ __global__ __launch_bounds__(256) void k_ov_hist(const T *__restrict__ b, Win w,
     const int rr = (int)(it / nchunk);
     const int r = rmin + rr;
     const int cb = (int)(it - (long)rr * nchunk) * kChunk, ce = min(cb + kChunk, w.w);
-    const T *brow = b + (long)r * w.w;
-    const T *mrow = mosaic + (long)(w.r0 + r) * w.ld + w.c0;
-    T o[4], v[4];
+    unsigned *krow = keys + (long)(r - w.o0) * w.w;
+    unsigned u4[4];
+    if (pass == 0) {
+      // keys of the overlap cells (0 elsewhere: no finite angle maps to 0),
+      // kept for the later passes
+      const T *brow = b + (long)r * w.w;
+      const T *mrow = mosaic + (long)(w.r0 + r) * w.ld + w.c0;
+      T o[4], v[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int c = cb + t + 256 * k;
-      o[k] = c < ce ? mrow[c] : T{};
-      v[k] = c < ce ? brow[c] : T{};
+      for (int k = 0; k < 4; ++k) {
+        const int c = cb + t + 256 * k;
+        o[k] = c < ce ? mrow[c] : T{};
+        v[k] = c < ce ? brow[c] : T{};
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int c = cb + t + 256 * k;
+        u4[k] = has_data(o[k]) && nonzero(v[k]) ? order_key(median_key(o[k], v[k], rot)) : 0u;
+        if (c < ce) krow[c] = u4[k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int c = cb + t + 256 * k;
+        u4[k] = c < ce ? krow[c] : 0u;
+      }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      if (!(has_data(o[k]) && nonzero(v[k]))) continue;
-      const unsigned u = order_key(median_key(o[k], v[k], rot));
+      const unsigned u = u4[k];
+      if (u == 0u) continue;
       const unsigned d = (u >> shift) & dmask;
       if ((u & hmask) == p0) atomicAdd(&h[0][d], 1u);
       if (!same && (u & hmask) == p1) atomicAdd(&h[1][d], 1u);
@@ -487,10 +509,26 @@ template <typename T>
 int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosaic, int left,
                 int top, int right, int bottom, double *offsets) {
   const int mh = bottom - top, mw = right - left;
+  // key buffer: one u32 per cell of the rows a burst shares with earlier
+  // bursts' boxes (the largest such band over the bursts)
+  size_t nkeys = 0;
+  {
+    int c0 = max(bursts[0].top, top), c1 = min(bursts[0].bottom, bottom);
+    for (int i = 1; i < n; ++i) {
+      const s1ifg_stitch_burst &bi = bursts[i];
+      const int h = bi.bottom - bi.top;
+      const int o0 = min(max(c0 - bi.top, 0), h), o1 = max(min(c1 - bi.top, h), o0);
+      if (bi.estimate) nkeys = std::max(nkeys, (size_t)(o1 - o0) * (size_t)(bi.right - bi.left));
+      c0 = min(c0, bi.top);
+      c1 = max(c1, bi.bottom);
+    }
+  }
+  const size_t ws_head = (sizeof(Workspace) + 255) & ~(size_t)255;
   void *p = nullptr;
-  int rc = ctx_workspace(ctx, sizeof(Workspace), &p);
+  int rc = ctx_workspace(ctx, ws_head + nkeys * sizeof(unsigned), &p);
   if (rc) return rc;
   Workspace *ws = (Workspace *)p;
+  unsigned *keys = (unsigned *)((char *)p + ws_head);
   hipStream_t s = ctx->stream;
   hipLaunchKernelGGL(k_init_state, dim3(1), dim3(256), 0, s, ws, offsets);
   S1_LAUNCH_CHECK("k_init_state");
@@ -515,7 +553,7 @@ int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosa
       S1_LAUNCH_CHECK("k_ov_partial");
       for (int pss = 0; pss < 3; ++pss) {
         hipLaunchKernelGGL(k_ov_hist<T>, dim3(kHistBlocks), dim3(256), 0, s, bd, w,
-                           (const T *)mosaic, ws, pss);
+                           (const T *)mosaic, ws, pss, keys);
         S1_LAUNCH_CHECK("k_ov_hist");
       }
       hipLaunchKernelGGL(k_ov_median, dim3(1), dim3(256), 0, s, ws);
