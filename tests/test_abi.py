@@ -38,6 +38,8 @@ def test_abi_version_and_errors():
     # argument validation happens before any device call
     rc = lib.s1ifg_ctx_set_stream(None, None)
     assert rc == -1
+    assert lib.s1ifg_stitch_subswath(None, None, 0, 0, None, 0, 0, 0, 0, None) == -1
+    assert lib.s1ifg_merge_subswath(None, None, 0, 0, 0, None, 0) == -1
     assert b"null context" in lib.s1ifg_last_error()
     rc = lib.s1ifg_geom_query(None, None, 0)
     assert rc == -1
@@ -69,7 +71,7 @@ def test_struct_layouts_match_c_compiler(tmp_path):
         pytest.skip("no C compiler")
     structs = {"s1ifg_orbit": N.Orbit, "s1ifg_burst": N.Burst, "s1ifg_grid": N.Grid,
                "s1ifg_dem": N.Dem, "s1ifg_pair_task": N.PairTask,
-               "s1ifg_batch_opts": N.BatchOpts}
+               "s1ifg_batch_opts": N.BatchOpts, "s1ifg_stitch_burst": N.StitchBurst}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "s1ifg.h"',
              "int main(void) {"]
     for cname, cls in structs.items():
