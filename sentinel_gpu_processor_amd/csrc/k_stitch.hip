@@ -30,6 +30,7 @@
 // do not depend on timing.
 #include <algorithm>
 #include <climits>
+#include <cstdint>
 
 #include "s1ifg_internal.h"
 
@@ -486,6 +487,56 @@ __global__ __launch_bounds__(256) void k_apply(const T *__restrict__ b, Win w,
   if (offsets && r == 0 && threadIdx.x == 0) offsets[burst] = (double)off;
 }
 
+// k_apply for complex bursts with 16-byte alignment (even window width,
+// column offset and row stride): two cells per 16-byte access.
+__global__ __launch_bounds__(256) void k_apply_c2(const float4 *__restrict__ b, Win w,
+                                                  float2 *__restrict__ mosaic,
+                                                  const Workspace *__restrict__ ws,
+                                                  double *__restrict__ offsets, int burst) {
+  const float off = ws->st.off;
+  const int r = blockIdx.x;
+  const int w2 = w.w >> 1;
+  const float4 *brow = b + (long)r * w2;
+  float2 *mrow = mosaic + (long)(w.r0 + r) * w.ld + w.c0;
+  float4 *mrow4 = reinterpret_cast<float4 *>(mrow);
+  const bool covered = r >= w.o0 && r < w.o1;
+  const float2 rot = unrotate(off);
+  constexpr int kU = 4;
+  for (int c0 = threadIdx.x; c0 < w2; c0 += 256 * kU) {
+    float4 v[kU];
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int c = c0 + 256 * k;
+      v[k] = c < w2 ? brow[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    float4 o[kU];
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int c = c0 + 256 * k;
+      o[k] = covered && c < w2 && (v[k].x != 0.f || v[k].z != 0.f) ? mrow4[c]
+                                                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int c = c0 + 256 * k;
+      const bool p0 = v[k].x != 0.f && o[k].x == 0.f;
+      const bool p1 = v[k].z != 0.f && o[k].z == 0.f;
+      float2 a = make_float2(v[k].x, v[k].y), bb = make_float2(v[k].z, v[k].w);
+      if (off != 0.f) {
+        a = cmul(a, rot);
+        bb = cmul(bb, rot);
+      }
+      if (p0 && p1) {
+        mrow4[c] = make_float4(a.x, a.y, bb.x, bb.y);
+      } else {
+        if (p0) mrow[2 * c] = a;
+        if (p1) mrow[2 * c + 1] = bb;
+      }
+    }
+  }
+  if (offsets && r == 0 && threadIdx.x == 0) offsets[burst] = (double)off;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_merge(const T *__restrict__ src, int h, int w,
                                                T *__restrict__ dst, long ld) {
@@ -559,9 +610,17 @@ int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosa
       hipLaunchKernelGGL(k_ov_median, dim3(1), dim3(256), 0, s, ws);
       S1_LAUNCH_CHECK("k_ov_median");
     }
-    hipLaunchKernelGGL(k_apply<T>, dim3(h), dim3(256), 0, s, bd, w, mosaic,
-                       (const Workspace *)ws, offsets, i);
-    S1_LAUNCH_CHECK("k_apply");
+    const bool pairs = sizeof(T) == 8 && w.w % 2 == 0 && w.c0 % 2 == 0 && w.ld % 2 == 0 &&
+                       (uintptr_t)bd % 16 == 0 && (uintptr_t)mosaic % 16 == 0;
+    if (pairs) {
+      hipLaunchKernelGGL(k_apply_c2, dim3(h), dim3(256), 0, s, (const float4 *)bd, w,
+                         (float2 *)mosaic, (const Workspace *)ws, offsets, i);
+      S1_LAUNCH_CHECK("k_apply_c2");
+    } else {
+      hipLaunchKernelGGL(k_apply<T>, dim3(h), dim3(256), 0, s, bd, w, mosaic,
+                         (const Workspace *)ws, offsets, i);
+      S1_LAUNCH_CHECK("k_apply");
+    }
     cov0 = min(cov0, bi.top);
     cov1 = max(cov1, bi.bottom);
   }
