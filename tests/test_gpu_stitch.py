@@ -181,3 +181,16 @@ def test_dropin_stitch_files(tmp_path, out_float):
     else:
         assert np.allclose(got, want, atol=4 * TOL, rtol=0)
     assert not any(os.path.exists(f) for f in files)
+
+
+def test_stitch_deterministic():
+    """Two runs give bit-identical mosaics and offsets (fixed-order FP64
+    partial sums, integer histograms)."""
+    rng = np.random.default_rng(16)
+    spans = [(k * 90, k * 90 + 120) for k in range(6)]
+    ids = [("x", str(k)) for k in range(6)]
+    bursts, lst = _make(rng, 1000, spans, rng.uniform(-3, 3, 6), ids, False)
+    a = _gpu(lst, ids, False)
+    b = _gpu(lst, ids, False)
+    assert np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64))
+    assert np.array_equal(a[2], b[2])
