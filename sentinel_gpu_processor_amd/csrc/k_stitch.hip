@@ -139,6 +139,23 @@ __global__ __launch_bounds__(256) void k_place_first(const T *__restrict__ b, in
   }
 }
 
+// k_place_first for complex bursts with even widths and column offset
+// (16-byte aligned rows): two cells per 16-byte access.
+__global__ __launch_bounds__(256) void k_place_first_c2(const float4 *__restrict__ b, int bh,
+                                                        int bw2, int br0, int bc02,
+                                                        float4 *__restrict__ mosaic, int mh,
+                                                        int mw2) {
+  const int r = blockIdx.x;
+  const int rb = r - br0;
+  const bool in_row = rb >= 0 && rb < bh;
+  for (int c = threadIdx.x; c < mw2; c += 256) {
+    const int cb = c - bc02;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (in_row && cb >= 0 && cb < bw2) v = b[(long)rb * bw2 + cb];
+    mosaic[(long)r * mw2 + c] = v;
+  }
+}
+
 // Fixed-order block sums: s_re / s_im / s_n added, s_lo / s_hi min / max.
 struct BlockSums {
   double re[256], im[256];
@@ -584,10 +601,18 @@ int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosa
   hipLaunchKernelGGL(k_init_state, dim3(1), dim3(256), 0, s, ws, offsets);
   S1_LAUNCH_CHECK("k_init_state");
   const s1ifg_stitch_burst &b0 = bursts[0];
-  hipLaunchKernelGGL(k_place_first<T>, dim3(mh), dim3(256), 0, s, (const T *)b0.data,
-                     b0.bottom - b0.top, b0.right - b0.left, b0.top - top, b0.left - left, mosaic,
-                     mh, mw);
-  S1_LAUNCH_CHECK("k_place_first");
+  const int bw0 = b0.right - b0.left, bc0 = b0.left - left;
+  if (sizeof(T) == 8 && mw % 2 == 0 && bw0 % 2 == 0 && bc0 % 2 == 0 &&
+      (uintptr_t)b0.data % 16 == 0 && (uintptr_t)mosaic % 16 == 0) {
+    hipLaunchKernelGGL(k_place_first_c2, dim3(mh), dim3(256), 0, s, (const float4 *)b0.data,
+                       b0.bottom - b0.top, bw0 / 2, b0.top - top, bc0 / 2, (float4 *)mosaic, mh,
+                       mw / 2);
+    S1_LAUNCH_CHECK("k_place_first_c2");
+  } else {
+    hipLaunchKernelGGL(k_place_first<T>, dim3(mh), dim3(256), 0, s, (const T *)b0.data,
+                       b0.bottom - b0.top, bw0, b0.top - top, bc0, mosaic, mh, mw);
+    S1_LAUNCH_CHECK("k_place_first");
+  }
   // rows of the mosaic that earlier bursts may have written (burst 0's box
   // clipped to the bounds, then the later bursts' boxes)
   int cov0 = max(b0.top, top), cov1 = min(b0.bottom, bottom);
