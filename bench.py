@@ -219,30 +219,6 @@ def run_gpu(args):
     return results
 
 
-def interior_mask(K, ctx, gr, gs, grid, dem_t, dem, dref, dsec, interp, offpoly, dh, rl, cl,
-                  win, margin=2):
-    """Multilooked pixels whose coherence window only sees target pixels
-    valid in both bursts with `margin` valid pixels around them (device
-    bool tensor): finite differences there cannot cross a validity edge."""
-    import torch
-    import torch.nn.functional as F
-    a = K.geocode(ctx, gr, grid, dem_t, dem, dref, interp, dh=dh)
-    b = K.geocode(ctx, gs, grid, dem_t, dem, dsec, interp, dh=dh, offpoly=offpoly)
-    bad = ((a == 0) | (b == 0)).float()[None, None]
-    del a, b
-    bad = F.max_pool2d(bad, 2 * margin + 1, stride=1, padding=margin)[0, 0] > 0
-    nr, nc = grid["nrow"] // rl, grid["ncol"] // cl
-    cell = ~bad[:nr * rl, :nc * cl].reshape(nr, rl, nc, cl).any(dim=3).any(dim=1)
-    h = win // 2
-    badc = F.max_pool2d((~cell).float()[None, None], 2 * h + 1, stride=1, padding=h)[0, 0] > 0
-    cell = ~badc
-    cell[:h, :] = False
-    cell[-h:, :] = False
-    cell[:, :h] = False
-    cell[:, -h:] = False
-    return cell
-
-
 def run_adjoint(args, ctx, gr, gs, grid, dem_t, dem, dref, dsec, offpoly, rl, cl):
     """C4: loss = -mean(5x5 coherence) over the interior of the C2 pair,
     gradient with respect to the 6 offset-polynomial coefficients and a
@@ -253,6 +229,7 @@ def run_adjoint(args, ctx, gr, gs, grid, dem_t, dem, dref, dsec, offpoly, rl, cl
     import torch
 
     from sentinel_gpu_processor_amd import kernels as K
+    from sentinel_gpu_processor_amd.adjoint import interior_cells
 
     dev = dref.device
     g = torch.Generator(device=dev)
@@ -260,8 +237,7 @@ def run_adjoint(args, ctx, gr, gs, grid, dem_t, dem, dref, dsec, offpoly, rl, cl
     dh = (torch.randn(grid["nrow"], grid["ncol"], device=dev, generator=g) * 5.0).contiguous()
     p = np.asarray(offpoly, np.float64)
     interp, win = args.interp, 5
-    mask = interior_mask(K, ctx, gr, gs, grid, dem_t, dem, dref, dsec, interp, p, dh, rl, cl,
-                         win)
+    mask = interior_cells(ctx, gr, gs, grid, dem_t, dem, dref, dsec, interp, p, dh, rl, cl, win)
     cnt = int(mask.sum())
 
     def forward(pp, hh):

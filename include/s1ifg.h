@@ -122,6 +122,16 @@ void s1ifg_ctx_destroy(s1ifg_ctx *ctx);
 /* Bind a HIP stream (hipStream_t as void*); NULL = the null stream. */
 int s1ifg_ctx_set_stream(s1ifg_ctx *ctx, void *stream);
 int s1ifg_ctx_sync(s1ifg_ctx *ctx);
+/* Validation hook (no reference counterpart): a DEVICE array of
+ * S1IFG_TILE_STATS uint64 counters to which later s1ifg_pair calls with
+ * sinc interpolation on this context add the classes of their 16 x 16-cell
+ * tiles: [0] tiles, [1] burst-tiles skipped by the tile's corner test,
+ * [2] burst-tiles interpolated from the LDS-staged tap window, [3]
+ * burst-tiles whose window exceeds the LDS budget (taps read from global
+ * memory), [4] burst-tiles not skipped without any valid pixel.  NULL
+ * turns counting off (the default).  The caller zeroes the array. */
+#define S1IFG_TILE_STATS 5
+int s1ifg_ctx_set_tile_stats(s1ifg_ctx *ctx, uint64_t *counts);
 /* Workspace currently held by the context, in bytes. */
 int s1ifg_ctx_workspace(s1ifg_ctx *ctx, int64_t *bytes);
 
@@ -137,6 +147,10 @@ void s1ifg_geom_destroy(s1ifg_geom *g);
  * out[5..8]=dc (t0,c0,c1,c2), out[9]=etac0, out[10]=tmid, out[11..13]=xmid,
  * out[14..16]=vmid.  n >= 17. */
 int s1ifg_geom_query(const s1ifg_geom *g, double *out, int n);
+/* The per-range-sample deramp vectors of the set-up, kt[j] and etaref[j]
+ * (geo2rdr.cu:533-542), as the kernels read them, into HOST arrays of n =
+ * samples doubles (either may be NULL).  Synchronous (validation only). */
+int s1ifg_geom_vectors(const s1ifg_geom *g, double *kt, double *etaref, int n);
 /* Burst footprint (latmin, latmax, lonmin, lonmax) for heights in
  * [hmin, hmax]; replaces bounds()/rah2ll (bounds.cu:237-317). HOST output. */
 int s1ifg_geom_bounds(const s1ifg_geom *g, double hmin, double hmax,

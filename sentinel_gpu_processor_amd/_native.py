@@ -19,6 +19,7 @@ GOLD_FP32 = 0
 GOLD_FP16_SPLIT = 1
 DEM_F32 = 0
 DEM_I16 = 1
+TILE_STATS = 5  # counters of s1ifg_ctx_set_tile_stats
 COH_FILL_CLAMP = 0
 COH_FILL_NAN = 1
 COH_FILL_HASH = 2
@@ -118,11 +119,13 @@ SIGNATURES = {
     "s1ifg_ctx_destroy": [_P],
     "s1ifg_ctx_set_stream": [_P, _P],
     "s1ifg_ctx_sync": [_P],
+    "s1ifg_ctx_set_tile_stats": [_P, _P],
     "s1ifg_ctx_workspace": [_P, ctypes.POINTER(_I64)],
     "s1ifg_geom_create": [_P, ctypes.POINTER(Orbit), ctypes.POINTER(Burst), ctypes.POINTER(_P)],
     "s1ifg_geom_destroy": [_P],
     "s1ifg_geom_query": [_P, ctypes.POINTER(_D), _I],
     "s1ifg_geom_bounds": [_P, _D, _D, ctypes.POINTER(_D)],
+    "s1ifg_geom_vectors": [_P, _P, _P, _I],
     "s1ifg_orbit_eval": [_P, _P, _P, _I64, _P, _P, _P],
     "s1ifg_deramp": [_P, _P, _P, _P],
     "s1ifg_geo2rdr": [_P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P, _P, _P],
@@ -231,6 +234,14 @@ class Context:
     def sync(self):
         check(self.lib.s1ifg_ctx_sync(self.handle))
 
+    def set_tile_stats(self, counts):
+        """Count the fused sinc pair kernel's tile classes into `counts` (a
+        zeroed int64 device tensor of TILE_STATS elements; None = off)."""
+        if counts is not None and (counts.numel() < TILE_STATS or counts.element_size() != 8):
+            raise ValueError(f"tile stats need {TILE_STATS} 64-bit counters")
+        self._tile_stats = counts
+        check(self.lib.s1ifg_ctx_set_tile_stats(self.handle, ptr(counts)))
+
     def workspace_bytes(self):
         v = ctypes.c_int64()
         check(self.lib.s1ifg_ctx_workspace(self.handle, ctypes.byref(v)))
@@ -313,6 +324,15 @@ class Geometry:
         v = list(out)
         return dict(ks=v[0], fm=v[1:5], dc=v[5:9], etac0=v[9], tmid=v[10], xmid=v[11:14],
                     vmid=v[14:17])
+
+    def vectors(self):
+        """(kt, etaref) per range sample, as the kernels read them."""
+        n = self.burst["samples"]
+        kt = np.empty(n, np.float64)
+        er = np.empty(n, np.float64)
+        check(self.ctx.lib.s1ifg_geom_vectors(self.handle, kt.ctypes.data_as(ctypes.c_void_p),
+                                              er.ctypes.data_as(ctypes.c_void_p), n))
+        return kt, er
 
     def bounds(self, hmin, hmax):
         out = (ctypes.c_double * 4)()

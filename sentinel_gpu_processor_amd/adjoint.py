@@ -83,3 +83,27 @@ def pair_coherence(ctx, g_ref, g_sec, grid, dem_t, dem, ref_d, sec_d, offpoly, d
     ifg, pr, ps = pair_products(ctx, g_ref, g_sec, grid, dem_t, dem, ref_d, sec_d, offpoly, dh,
                                 interp, rl, cl)
     return CoherenceFunction.apply(ifg, pr, ps, ctx, win, fill, seed)
+
+
+def interior_cells(ctx, g_ref, g_sec, grid, dem_t, dem, ref_d, sec_d, interp, offpoly, dh, rl,
+                   cl, win, margin=2):
+    """Multilooked pixels whose coherence window sees only target pixels
+    valid in both bursts with `margin` valid pixels around them (device bool
+    tensor): finite differences of the loss there cannot cross a validity
+    edge, where the products jump."""
+    import torch.nn.functional as F
+    a = K.geocode(ctx, g_ref, grid, dem_t, dem, ref_d, interp, dh=dh)
+    b = K.geocode(ctx, g_sec, grid, dem_t, dem, sec_d, interp, dh=dh, offpoly=offpoly)
+    bad = ((a == 0) | (b == 0)).float()[None, None]
+    del a, b
+    bad = F.max_pool2d(bad, 2 * margin + 1, stride=1, padding=margin)[0, 0] > 0
+    nr, nc = grid["nrow"] // rl, grid["ncol"] // cl
+    cell = ~bad[:nr * rl, :nc * cl].reshape(nr, rl, nc, cl).any(dim=3).any(dim=1)
+    h = win // 2
+    badc = F.max_pool2d((~cell).float()[None, None], 2 * h + 1, stride=1, padding=h)[0, 0] > 0
+    cell = ~badc
+    cell[:h, :] = False
+    cell[-h:, :] = False
+    cell[:, :h] = False
+    cell[:, -h:] = False
+    return cell

@@ -319,6 +319,7 @@ int s1ifg_ctx_create(int device, s1ifg_ctx **out) {
   c->ws = nullptr;
   c->ws_bytes = 0;
   c->ws_limit = -1;
+  c->tile_stats = nullptr;
   const char *lim = std::getenv("S1IFG_WORKSPACE_LIMIT");
   if (lim && *lim) c->ws_limit = std::atoll(lim);
   *out = c;
@@ -339,6 +340,12 @@ void s1ifg_ctx_destroy(s1ifg_ctx *ctx) {
 int s1ifg_ctx_set_stream(s1ifg_ctx *ctx, void *stream) {
   if (!ctx) return set_error(S1IFG_EINVAL, "null context");
   ctx->stream = (hipStream_t)stream;
+  return S1IFG_OK;
+}
+
+int s1ifg_ctx_set_tile_stats(s1ifg_ctx *ctx, uint64_t *counts) {
+  if (!ctx) return set_error(S1IFG_EINVAL, "null context");
+  ctx->tile_stats = reinterpret_cast<unsigned long long *>(counts);
   return S1IFG_OK;
 }
 
@@ -545,6 +552,20 @@ int s1ifg_geom_query(const s1ifg_geom *g, double *out, int n) {
     out[11 + k] = d.xmid[k];
     out[14 + k] = d.vmid[k];
   }
+  return S1IFG_OK;
+}
+
+int s1ifg_geom_vectors(const s1ifg_geom *g, double *kt, double *etaref, int n) {
+  if (!g || (!kt && !etaref)) return set_error(S1IFG_EINVAL, "need a geometry and an output");
+  if (n != g->dev.samples)
+    return set_error(S1IFG_EINVAL, "n = %d, the burst has %d samples", n, g->dev.samples);
+  int dev = -1;
+  S1_HIP(hipGetDevice(&dev));
+  if (dev != g->device) S1_HIP(hipSetDevice(g->device));
+  // the device copies the kernels read (uploaded once when the geometry was built)
+  if (kt) S1_HIP(hipMemcpy(kt, g->dev.kt, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  if (etaref)
+    S1_HIP(hipMemcpy(etaref, g->dev.etaref, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
   return S1IFG_OK;
 }
 

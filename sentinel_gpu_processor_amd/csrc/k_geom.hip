@@ -96,10 +96,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr,
     DemDev dem, const RowTab *rows, const ColTab *cols, const float *dh,
     const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl, int cl, int nr_ml,
-    int nc_ml, float2 *__restrict__ ifg, float *__restrict__ pr, float *__restrict__ ps) {
+    int nc_ml, float2 *__restrict__ ifg, float *__restrict__ pr, float *__restrict__ ps,
+    unsigned long long *stats) {
   __shared__ PairLds L;
   pair_tile_sinc(*gpr, *gps, ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, blockIdx.y,
-                 blockIdx.x, nr_ml, nc_ml, ifg, pr, ps, L);
+                 blockIdx.x, nr_ml, nc_ml, ifg, pr, ps, L, stats);
 }
 
 __global__ void k_rows_init(int *fl, int nrow) {
@@ -295,7 +296,7 @@ int s1ifg_pair(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
     hipLaunchKernelGGL(k_pair_tiles, tg, dim3(256), 0, ctx->stream, g_ref->d_dev, g_sec->d_dev,
                        op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
                        (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
-                       pow_sec);
+                       pow_sec, ctx->tile_stats);
   } else if (interp == S1IFG_INTERP_BILINEAR)
     hipLaunchKernelGGL(k_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, ctx->stream, g_ref->d_dev,
                        g_sec->d_dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,

@@ -72,6 +72,13 @@ __device__ __forceinline__ int wave_max_i(int v) {
   return wave_reduce(v, [](int a, int b) { return max(a, b); });
 }
 
+// Tile classes counted into the optional stats array (validation only):
+// [0] tiles, [1] burst-tiles skipped by the corner test, [2] burst-tiles
+// interpolated from the LDS window, [3] burst-tiles read from global memory
+// (window larger than kPairWin), [4] burst-tiles not skipped whose pixels
+// all lie outside the validity window.
+enum { kStatTiles, kStatSkipped, kStatStaged, kStatGlobal, kStatEmpty, kStatCount };
+
 // One tile (tile_y, tile_x) of look cells.  All 256 threads of the block
 // call it (block barriers); rl * cl <= kPairMaxPix.  A block that runs
 // another tile afterwards must pass a block barrier first (L is reused).
@@ -79,7 +86,8 @@ __device__ __forceinline__ void pair_tile_sinc(
     const GeomDev &G0, const GeomDev &G1, const OffPoly &op1, const GridDev &gr,
     const DemDev &dem, const RowTab *rows, const ColTab *cols, const float *dh,
     const float2 *__restrict__ z0, const float2 *__restrict__ z1, int rl, int cl, int tile_y,
-    int tile_x, int nr_ml, int nc_ml, float2 *ifg, float *pr, float *ps, PairLds &L) {
+    int tile_x, int nr_ml, int nc_ml, float2 *ifg, float *pr, float *ps, PairLds &L,
+    unsigned long long *stats = nullptr) {
   const OffPoly none = {{0, 0, 0, 0, 0, 0}, 0};
   const int tid = threadIdx.x;
   const int cm = tile_x * kPairTile + (tid & (kPairTile - 1));
@@ -100,6 +108,10 @@ __device__ __forceinline__ void pair_tile_sinc(
   }
   __syncthreads();  // box initialised, L.out set
   const bool out0 = L.out[0], out1 = L.out[1];
+  if (stats && tid == 0) {
+    atomicAdd(&stats[kStatTiles], 1ull);
+    atomicAdd(&stats[kStatSkipped], (unsigned long long)(out0 + out1));
+  }
   if (out0 && out1) {
     // no pixel of the tile in either burst: the products are zero
     if (active) {
@@ -169,6 +181,8 @@ __device__ __forceinline__ void pair_tile_sinc(
     const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
     const bool staged = any && W * H <= kPairWin;
     const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
+    if (stats && tid == 0 && !out0)
+      atomicAdd(&stats[staged ? kStatStaged : any ? kStatGlobal : kStatEmpty], 1ull);
     if (staged) stage_window(z0, G0.samples, wa0, wr0, wc, (int)H, L.win);
     __syncthreads();
 #pragma unroll
@@ -234,6 +248,8 @@ __device__ __forceinline__ void pair_tile_sinc(
     const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
     const bool staged = any && W * H <= kPairWin;
     const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
+    if (stats && tid == 0 && !out1)
+      atomicAdd(&stats[staged ? kStatStaged : any ? kStatGlobal : kStatEmpty], 1ull);
     if (staged) stage_window(z1, G1.samples, wa0, wr0, wc, (int)H, L.win);
     __syncthreads();
 #pragma unroll

@@ -89,6 +89,7 @@ struct s1ifg_ctx {
   void *ws;            // workspace
   size_t ws_bytes;
   int64_t ws_limit;    // optional cap (S1IFG_WORKSPACE_LIMIT env), -1 = none
+  unsigned long long *tile_stats;  // optional device counters of the pair kernel's tile classes
 };
 
 struct s1ifg_geom {

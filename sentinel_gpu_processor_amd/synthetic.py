@@ -20,7 +20,8 @@ The scene is physically consistent with the processing chain:
 ``fidelity="exact"`` evaluates the secondary mapping per pixel (FP64) and
 resamples z with a 12-tap windowed sinc; ``fidelity="fast"`` evaluates the
 mapping on a coarse grid and resamples on the GPU with torch (used for the
-large benchmark bursts, where only the data volume and geometry matter).
+large benchmark bursts, where only the data volume and geometry matter);
+``fidelity="none"`` builds the orbits, bursts, grid and DEM only.
 """
 
 import math
@@ -173,24 +174,78 @@ def rdr2geo(orbit, t, R, dem, iters=8):
     return point(theta)
 
 
-def deramp_coefficients(orbit, burst):
-    """kt[N], etaref[N], eta[L] of the burst (geo2rdr.cu:477-542) for a
-    straight-line orbit and constant FM/DC records (the synthetic scenes)."""
-    L, N = burst["lines"], burst["samples"]
-    ati = burst["azimuth_time_interval"]
+def record_at(rec, tc):
+    """Coefficients (t0, c0, c1, c2) of the FM-rate / Doppler records
+    rec[k] = (t, t0, c0, c1, c2) at time tc: linear interpolation between the
+    two records around tc, the first (last) record before (after) them.
+    Records are sorted by time."""
+    rec = np.asarray(rec, np.float64)
+    return np.array([np.interp(tc, rec[:, 0], rec[:, 1 + k]) for k in range(4)])
+
+
+def tops_rates(orbit, burst, tau):
+    """TOPS rates of a straight-line-orbit burst at two-way range times tau:
+    (kt, etaref).  ka and the Doppler centroid are the records' polynomials
+    at the burst centre time; ks = 2 |v| f0 / c * steering rate;
+    kt = ka ks / (ka - ks); the Doppler-centroid time etac = -fdc / ka is
+    referred to its value at the first range sample."""
+    L = burst["lines"]
+    tc = burst["t_start"] + L * burst["azimuth_time_interval"] / 2
     vs = float(np.linalg.norm(orbit.v))
     ks = 2.0 * vs * burst["radar_frequency"] / SOL * burst["azimuth_steering_rate"] * np.pi / 180.0
-    fm = np.asarray(burst["fm"])[0, 1:]
-    dc = np.asarray(burst["dc"])[0, 1:]
+    fm = record_at(burst["fm"], tc)
+    dc = record_at(burst["dc"], tc)
+
+    def etac_ka(t):
+        ka = np.polyval([fm[3], fm[2], fm[1]], t - fm[0])
+        fdc = np.polyval([dc[3], dc[2], dc[1]], t - dc[0])
+        return -fdc / ka, ka
+
+    etac, ka = etac_ka(np.asarray(tau, np.float64))
+    etac0, _ = etac_ka(burst["slant_range_time"])
+    return ka * ks / (ka - ks), etac - etac0
+
+
+def tops_records(t_center, tau_c, kind="constant", shift=None):
+    """FM-rate and Doppler-centroid records (t, t0, c0, c1, c2), three each,
+    around a burst centred at t_center with mid-swath two-way range time
+    tau_c.
+
+    "constant": ka = -2300 Hz/s, fdc = 0, the records at t_center + 2.75 k.
+    "varying": ka = c0 + c1 dt + c2 dt^2 with c0 ~ -2300 Hz/s,
+    c1 ~ 4.5e5 Hz/s^2, c2 ~ -7.5e7 Hz/s^3 and fdc with c0 ~ 25 Hz,
+    c1 ~ -4e4 Hz/s, c2 ~ 8e6 Hz/s^2 (dt = tau - t0), every coefficient and
+    t0 different per record; the FM records are 0.9 s and the Doppler
+    records -1.3 s off t_center + 2.75 k, so the burst centre falls between
+    two records of each.  shift adds a time offset to every record (to put
+    the centre before the first or after the last record)."""
+    ks = (-1, 0, 1)
+    if kind == "constant":
+        fm = np.array([[t_center + k * 2.75, tau_c, FM_RATE, 0.0, 0.0] for k in ks])
+        dc = np.array([[t_center + k * 2.75, tau_c, 0.0, 0.0, 0.0] for k in ks])
+    elif kind == "varying":
+        fm = np.array([[t_center + 0.9 + 2.75 * k, tau_c + 2e-7 * k, FM_RATE + 15.0 * k,
+                        4.5e5 + 2.0e4 * k, -7.5e7 + 5.0e6 * k] for k in ks])
+        dc = np.array([[t_center - 1.3 + 2.75 * k, tau_c - 1e-7 * k, 25.0 - 4.0 * k,
+                        -4.0e4 + 3.0e3 * k, 8.0e6 - 1.0e6 * k] for k in ks])
+    else:
+        raise ValueError(f"tops kind {kind!r}")
+    if shift:
+        fm[:, 0] += shift
+        dc[:, 0] += shift
+    return fm, dc
+
+
+def deramp_coefficients(orbit, burst):
+    """kt[N], etaref[N], eta[L] of the burst (the quantities of
+    geo2rdr.cu:477-542) for a straight-line orbit: tops_rates at the range
+    samples, eta = -L ati / 2 + i ati."""
+    L, N = burst["lines"], burst["samples"]
+    ati = burst["azimuth_time_interval"]
     eta = -L * ati / 2. + np.arange(L) * ati
     tr = burst["slant_range_time"] + np.arange(N) / burst["range_sampling_rate"]
-    dt = tr - fm[0]
-    ka = fm[1] + dt * (fm[2] + fm[3] * dt)
-    dt = tr - dc[0]
-    fnc = dc[1] + dt * (dc[2] + dc[3] * dt)
-    kt = ka * ks / (ka - ks)
-    etac = -fnc / ka
-    return kt, etac - etac[0], eta
+    kt, etaref = tops_rates(orbit, burst, tr)
+    return kt, etaref, eta
 
 
 def ramp(orbit, burst, rows=None, cols=None):
@@ -272,15 +327,19 @@ def iw_sizes(seed=0, n_sub=3, n_burst=9, lines=(1450, 1530), samples=(20000, 255
 
 def make_iw(seed=0, n_sub=3, n_burst=9, lines=(1450, 1530), samples=(20000, 25500),
             dem_shape=(2000, 3000), dem_kind="fractal", hmax=2500.0, misreg=(0.3, 0.05),
-            fidelity="fast", device=None):
+            fidelity="fast", device=None, select=None):
     """Synthetic IW pair: n_sub subswaths x n_burst bursts of uneven size.
     Burst k of subswath s is centred 0.17 deg further along track than burst
     k-1 (about one burst length) and subswath s looks 50 km further in
-    range than s-1; each burst pair has its own DEM window and seed."""
+    range than s-1; each burst pair has its own DEM window and seed.
+    select: indices (s * n_burst + k) of the bursts to build (default all);
+    each is the same as in the whole IW pair."""
     sizes = iw_sizes(seed, n_sub, n_burst, lines, samples)
     out = []
     for s in range(n_sub):
         for k in range(n_burst):
+            if select is not None and s * n_burst + k not in select:
+                continue
             L, N = sizes[s * n_burst + k]
             sc = make_scene(lines=L, samples=N, dem_shape=dem_shape, dem_kind=dem_kind,
                             hmax=hmax, seed=seed * 1000 + s * n_burst + k, misreg=misreg,
@@ -370,8 +429,13 @@ def make_scene(lines=512, samples=2048, dem_shape=(256, 256), dem_kind="hill", h
                seed=0, gamma0=0.8, phase_noise=0.5, amb_height=500.0, heading=-10.0,
                look_range=800.0e3, baseline=(120.0, 30.0), misreg=None, misreg_ramp=(0.0, 0.0),
                fidelity="exact", device=None, lat_c=34.0, lon_c=-118.0, t_center=43200.0,
-               margin=24, sec_seed=None, aps_sec=None):
+               margin=24, sec_seed=None, aps_sec=None, tops="constant"):
     """Build one synthetic burst pair.
+
+    tops: "constant" (FM rate -2300 Hz/s, zero Doppler centroid, three equal
+    records) or "varying" (range-dependent FM rate and Doppler centroid whose
+    records change with time, the burst centre between two records): see
+    tops_records.
 
     misreg: (range px, azimuth px) constant misregistration of the secondary
     content (corrected by the offset polynomial p = [drg, rg_ramp, 0, daz,
@@ -404,8 +468,7 @@ def make_scene(lines=512, samples=2048, dem_shape=(256, 256), dem_kind="hill", h
     t_end = t_start + (L - 1) * AZ_INTERVAL
     n_sv = int(math.ceil(t_end - t_start)) + 12
     t0 = math.floor(t_start) - 6
-    fm = np.array([[t_center + k * 2.75, 2 * Rc / SOL, FM_RATE, 0.0, 0.0] for k in (-1, 0, 1)])
-    dc = np.array([[t_center + k * 2.75, 2 * Rc / SOL, 0.0, 0.0, 0.0] for k in (-1, 0, 1)])
+    fm, dc = tops_records(t_center, 2 * Rc / SOL, tops)
     flv = max(2, L // 50)
     fvs = max(2, N // 100)
     burst = dict(lines=L, samples=N, first_valid_line=flv, last_valid_line=L - flv,
@@ -454,6 +517,8 @@ def make_scene(lines=512, samples=2048, dem_shape=(256, 256), dem_kind="hill", h
     out = dict(orbit_ref=orbit_ref, orbit_sec=orbit_sec, burst_ref=burst, burst_sec=burst_sec,
                grid=grid, dem=dem, offpoly=offpoly, gamma0=gamma0, amb_height=amb_height,
                line_orbits=(orb_ref, orb_sec))
+    if fidelity == "none":
+        return out
     if fidelity == "exact":
         ref, sec = _slcs_exact(rng, orb_ref, orb_sec, burst, dem, gamma0, phase_noise,
                                amb_height, offpoly, margin, sec_seed)

@@ -22,6 +22,14 @@ def scene(name="small", seed=0):
     if name == "wide":
         return S.make_scene(lines=150, samples=640, dem_shape=(48, 80), hmax=250.0, seed=seed,
                             misreg=(-0.2, 0.1))
+    if name == "small_tops":
+        # range-dependent FM rate and Doppler centroid, time-varying records
+        return S.make_scene(lines=128, samples=512, dem_shape=(64, 64), hmax=300.0, seed=seed,
+                            misreg=(0.3, 0.05), misreg_ramp=(0.2, 0.1), tops="varying")
+    if name == "long_tops":
+        # 1024 lines: the TOPS ramps span +-1.05 s of azimuth time
+        return S.make_scene(lines=1024, samples=512, dem_shape=(96, 64), hmax=300.0, seed=seed,
+                            misreg=(0.3, 0.05), misreg_ramp=(0.2, 0.1), tops="varying")
     if name == "c1":
         return S.make_scene(**_c1_kwargs(), seed=seed)
     raise KeyError(name)
@@ -32,11 +40,17 @@ def _c1_kwargs():
     return dict(lines=c["lines"], samples=c["samples"], dem_shape=c["dem"], hmax=c["hmax"])
 
 
+def dem_i16(dem):
+    """The DEM as the reference stores it: int16 metres (rounded)."""
+    return dict(dem, data=np.round(np.asarray(dem["data"], np.float64)).astype(np.int16))
+
+
 @functools.lru_cache(maxsize=None)
-def oracle_chain(name="small", interp="bilinear", seed=0, rl=1, cl=4):
+def oracle_chain(name="small", interp="bilinear", seed=0, rl=1, cl=4, dem16=False):
     sc = scene(name, seed)
     b = sc["burst_ref"]
-    h = G.target_heights(sc["grid"], sc["dem"])
+    dem = dem_i16(sc["dem"]) if dem16 else sc["dem"]
+    h = G.target_heights(sc["grid"], dem)
     su = G.deramp_setup(sc["orbit_ref"], b)
     ss = G.deramp_setup(sc["orbit_sec"], sc["burst_sec"])
     dr = G.deramp(sc["ref_slc"], su)

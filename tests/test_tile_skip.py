@@ -96,3 +96,66 @@ def test_tile_corner_bound(name):
     assert worst < MARGIN / 8, worst
     print(name, "worst excess", worst, "skipped", skipped, "of", tiles)
     assert skipped > 0
+
+
+def _tile_corners(grid, dem, nrow, ncol):
+    """Per tile (row-major over the tile grid): the 8 corner (row, column,
+    height) points of the skip test."""
+    rs = list(range(0, nrow, TILE_R))
+    cs = list(range(0, ncol, TILE_C))
+    pts = []
+    for r0 in rs:
+        r1 = min(r0 + TILE_R, nrow) - 1
+        for c0 in cs:
+            c1 = min(c0 + TILE_C, ncol) - 1
+            lo, hi = _tile_heights(grid, dem, r0, r1, c0, c1)
+            pts.append((r0, r1, c0, c1, lo, hi))
+    P = np.array(pts)
+    cr = np.stack([P[:, 0], P[:, 0], P[:, 1], P[:, 1]] * 2, axis=1)
+    cc = np.stack([P[:, 2], P[:, 3], P[:, 2], P[:, 3]] * 2, axis=1)
+    ch = np.concatenate([np.repeat(P[:, 4:5], 4, 1), np.repeat(P[:, 5:6], 4, 1)], axis=1)
+    return P.astype(np.int64), cr, cc, ch
+
+
+@pytest.mark.parametrize("hmax", [2500.0, 3000.0])
+def test_tile_corner_bound_high_relief(hmax):
+    """The same measurement on the benchmark's burst geometry (1500 x 25000,
+    misregistered secondary) with fractal relief of 2500 and 3000 m, over the
+    west 1500 x 2048 pixels of the grid, where the footprint's edge crosses
+    the grid (tiles outside, inside and straddling it)."""
+    from sentinel_gpu_processor_amd import synthetic as S
+    sc = S.make_scene(lines=1500, samples=25000, dem_shape=(2000, 3000), dem_kind="fractal",
+                      hmax=hmax, seed=0, misreg=(0.3, 0.05), misreg_ramp=(0.2, 0.1),
+                      fidelity="none")
+    grid = dict(sc["grid"], ncol=2048)
+    dem = sc["dem"]
+    h = G.target_heights(grid, dem)
+    assert np.ptp(dem["data"]) > 0.95 * hmax
+    nrow, ncol = grid["nrow"], grid["ncol"]
+    P, cr, cc, ch = _tile_corners(grid, dem, nrow, ncol)
+    worst = -np.inf
+    skipped = 0
+    for orbit, burst, op in ((sc["orbit_ref"], sc["burst_ref"], None),
+                             (sc["orbit_sec"], sc["burst_sec"], sc["offpoly"])):
+        rr, ccol = np.meshgrid(np.arange(nrow), np.arange(ncol), indexing="ij")
+        rg, az = _positions(orbit, burst, grid, rr, ccol, h, op)
+        valid = ((rg >= burst["first_valid_sample"] + 3) & (rg < burst["last_valid_sample"] - 4)
+                 & (az >= burst["first_valid_line"] + 5) & (az < burst["last_valid_line"] - 5))
+        krg, kaz = _positions(orbit, burst, grid, cr, cc, ch, op)
+        lr = burst["first_valid_sample"] + 3
+        ur = burst["last_valid_sample"] - 4
+        la = burst["first_valid_line"] + 5
+        ua = burst["last_valid_line"] - 5
+        for t, (r0, r1, c0, c1, _, _) in enumerate(P):
+            prg = rg[r0:r1 + 1, c0:c1 + 1]
+            paz = az[r0:r1 + 1, c0:c1 + 1]
+            for pv, kv in ((prg, krg[t]), (-prg, -krg[t]), (paz, kaz[t]), (-paz, -kaz[t])):
+                worst = max(worst, float(pv.max() - kv.max()))
+            out = ((krg[t] < lr - MARGIN).all() or (krg[t] >= ur + MARGIN).all()
+                   or (kaz[t] < la - MARGIN).all() or (kaz[t] >= ua + MARGIN).all())
+            if out:
+                skipped += 1
+                assert not valid[r0:r1 + 1, c0:c1 + 1].any()
+    print(hmax, "worst excess", worst, "skipped", skipped, "of", 2 * len(P))
+    assert worst < MARGIN / 8, worst
+    assert skipped > 0.2 * 2 * len(P)
