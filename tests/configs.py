@@ -16,13 +16,14 @@ from sentinel_gpu_processor_amd import synthetic as S
 
 
 @functools.lru_cache(maxsize=2)
-def c2_scene(tops="constant"):
+def c2_scene(tops="constant", baseline=(120.0, 30.0)):
     """The 1500 x 25000 pair: fractal DEM 2000 x 3000 (0-2500 m), secondary
     misregistered by (0.3, 0.05) px plus ramps (0.2, 0.1), SLCs on cuda:0."""
     c = S.CONFIGS["c2"]
     return S.make_scene(lines=c["lines"], samples=c["samples"], dem_shape=c["dem"],
                         dem_kind=c["dem_kind"], hmax=c["hmax"], seed=0, misreg=c["offset"],
-                        misreg_ramp=(0.2, 0.1), fidelity="fast", device="cuda:0", tops=tops)
+                        misreg_ramp=(0.2, 0.1), fidelity="fast", device="cuda:0", tops=tops,
+                        baseline=baseline)
 
 
 def crop_grid(grid, r0, c0, nr, nc):
