@@ -254,6 +254,21 @@ int s1ifg_pair(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
                const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl,
                s1ifg_cf32 *ifg_ml, float *pow_ref, float *pow_sec);
 
+/* s1ifg_pair without any device allocation: the same kernels and results,
+ * with the grid tables (the call's only scratch memory) in a fixed 2 MiB
+ * device array of the library instead of the context workspace.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Limit: 32 * nrow + 24 * ncol bytes (each rounded up to 256)
+ * <= 2 MiB, e.g. 16384 x 65536; larger grids fail with S1IFG_ERANGE.  Every
+ * call on a device shares that device's array: calls that overlap on one
+ * device race (serialise them, e.g. on one stream). */
+int s1ifg_pair_noalloc(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
+                       const s1ifg_grid *grid, const s1ifg_dem *dem, const float *dh,
+                       const double *offpoly_sec, const s1ifg_cf32 *ref_deramped,
+                       const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl,
+                       s1ifg_cf32 *ifg_ml, float *pow_ref, float *pow_sec);
+
 /* ------------------------------------------------------------ adjoints */
 /* Adjoint of s1ifg_pair (no reference counterpart).  Given the gradients of
  * a real loss with respect to the pair's outputs on the multilooked grid --

@@ -13,6 +13,7 @@ import numpy as np
 from . import _build
 
 S1IFG_OK = 0
+S1IFG_ERANGE = -4
 INTERP_BILINEAR = 0
 INTERP_SINC = 1
 GOLD_FP32 = 0
@@ -139,6 +140,8 @@ SIGNATURES = {
     "s1ifg_goldstein": [_P, _P, _I, _I, _I, _F, _I, _P],
     "s1ifg_pair": [_P, _P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P, _P, _I,
                    _I, _I, _P, _P, _P],
+    "s1ifg_pair_noalloc": [_P, _P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P,
+                           _P, _I, _I, _I, _P, _P, _P],
     "s1ifg_pair_backward": [_P, _P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P,
                             _P, _I, _I, _I, _P, _P, _P, _P, _P],
     "s1ifg_coherence_backward": [_P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P],

@@ -1,5 +1,7 @@
 // Geometry-path kernels: grid tables, orbit evaluation, deramp, geo2rdr,
 // geocode, valid-row scan, and the fused burst-pair kernel.
+#include <mutex>
+
 #include "geom_dev.h"
 #include "pair_lds_dev.h"
 
@@ -136,6 +138,26 @@ __global__ void k_rows_final(int *fl, int nrow) {
   }
 }
 
+// Byte offsets of the column table and of what follows the tables.
+size_t grid_tables_col_offset(const GridDev &gr) {
+  return (sizeof(RowTab) * (size_t)gr.nrow + 255) & ~(size_t)255;
+}
+size_t grid_tables_bytes(const GridDev &gr) {
+  return (grid_tables_col_offset(gr) + sizeof(ColTab) * (size_t)gr.ncol + 255) & ~(size_t)255;
+}
+
+// Fill the grid tables at base (grid_tables_bytes(gr) bytes of device memory).
+int grid_tables_at(s1ifg_ctx *ctx, const GridDev &gr, const DemDev &dem, void *base,
+                   RowTab **rows, ColTab **cols) {
+  *rows = (RowTab *)base;
+  *cols = (ColTab *)((char *)base + grid_tables_col_offset(gr));
+  int n = gr.nrow > gr.ncol ? gr.nrow : gr.ncol;
+  hipLaunchKernelGGL(k_grid_tables, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, gr, dem,
+                     *rows, *cols);
+  S1_LAUNCH_CHECK("k_grid_tables");
+  return S1IFG_OK;
+}
+
 }  // namespace
 
 namespace s1ifg {
@@ -144,22 +166,50 @@ namespace s1ifg {
 // scratch for the caller (256-byte aligned; *extra_ptr may be NULL).
 int grid_tables(s1ifg_ctx *ctx, const GridDev &gr, const DemDev &dem, RowTab **rows,
                 ColTab **cols, size_t extra, void **extra_ptr) {
-  size_t off_c = (sizeof(RowTab) * (size_t)gr.nrow + 255) & ~(size_t)255;
-  size_t off_x = (off_c + sizeof(ColTab) * (size_t)gr.ncol + 255) & ~(size_t)255;
+  const size_t off_x = grid_tables_bytes(gr);
   void *ws = nullptr;
   int rc = ctx_workspace(ctx, off_x + extra + 256, &ws);
   if (rc) return rc;
-  *rows = (RowTab *)ws;
-  *cols = (ColTab *)((char *)ws + off_c);
   if (extra_ptr) *extra_ptr = (char *)ws + off_x;
-  int n = gr.nrow > gr.ncol ? gr.nrow : gr.ncol;
-  hipLaunchKernelGGL(k_grid_tables, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, gr, dem,
-                     *rows, *cols);
-  S1_LAUNCH_CHECK("k_grid_tables");
-  return S1IFG_OK;
+  return grid_tables_at(ctx, gr, dem, ws, rows, cols);
 }
 
 }  // namespace s1ifg
+
+namespace {
+
+// Device memory of the allocation-free pair entry point: the grid tables of
+// one call, in a file-scope device array of fixed size.  Each device holds
+// its own copy for the life of the process; host code reaches it only
+// through the address hipGetSymbolAddress returns on that device (the
+// array's name is a host-side stand-in, not a device address).  Calls that
+// overlap on one device share it and race.
+constexpr size_t kStaticScratchBytes = (size_t)2 << 20;  // 2 MiB
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices];  // device address of `scratch`, per device
+
+// The device address of `scratch` on the context's device (current after
+// check_ctx), looked up on the first call on that device and kept.
+int static_scratch(s1ifg_ctx *ctx, void **out) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return set_error(S1IFG_EINVAL, "device %d beyond the %d this entry point supports",
+                     ctx->device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[ctx->device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_error(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return set_error(S1IFG_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_dev[ctx->device] = p;
+  }
+  *out = g_scratch_dev[ctx->device];
+  return S1IFG_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -177,6 +227,66 @@ int common_setup(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_grid *grid,
   if ((rc = make_grid(grid, gr))) return rc;
   if ((rc = make_dem(dem, dd))) return rc;
   if ((rc = make_offpoly(offpoly, op))) return rc;
+  return S1IFG_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// The fused burst pair with its grid tables in the context workspace
+// (static_tables = false), or in the file-scope `scratch` array
+// (static_tables = true: nothing is allocated).
+int pair_impl(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
+              const s1ifg_grid *grid, const s1ifg_dem *dem, const float *dh,
+              const double *offpoly_sec, const s1ifg_cf32 *ref_deramped,
+              const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl, s1ifg_cf32 *ifg_ml,
+              float *pow_ref, float *pow_sec, bool static_tables) {
+  GridDev gr;
+  DemDev dd;
+  OffPoly op;
+  int rc = common_setup(ctx, g_ref, grid, dem, offpoly_sec, &gr, &dd, &op);
+  if (rc) return rc;
+  if ((rc = check_geom(ctx, g_sec))) return rc;
+  if (!ref_deramped || !sec_deramped || !ifg_ml) return set_error(S1IFG_EINVAL, "null data pointer");
+  if (rl < 1 || cl < 1) return set_error(S1IFG_EINVAL, "looks %d x %d", rl, cl);
+  if (interp != S1IFG_INTERP_BILINEAR && interp != S1IFG_INTERP_SINC)
+    return set_error(S1IFG_EINVAL, "interp %d", interp);
+  int nr = gr.nrow / rl, nc = gr.ncol / cl;
+  if (nr < 1 || nc < 1) return set_error(S1IFG_EINVAL, "grid smaller than one look cell");
+  RowTab *rows;
+  ColTab *cols;
+  if (static_tables) {
+    const size_t need = grid_tables_bytes(gr);
+    if (need > kStaticScratchBytes)
+      return set_error(S1IFG_ERANGE,
+                       "grid %d x %d needs %zu bytes of tables, beyond the %zu bytes of the "
+                       "allocation-free pair (32 * nrow + 24 * ncol, each rounded up to 256)",
+                       gr.nrow, gr.ncol, need, kStaticScratchBytes);
+    void *base = nullptr;
+    if ((rc = static_scratch(ctx, &base))) return rc;
+    if ((rc = grid_tables_at(ctx, gr, dd, base, &rows, &cols))) return rc;
+  } else if ((rc = grid_tables(ctx, gr, dd, &rows, &cols))) {
+    return rc;
+  }
+  dim3 blk(64, 4), grd((nc + 63) / 64, (nr + 3) / 4);
+  if (interp == S1IFG_INTERP_SINC && rl * cl <= kPairMaxPix) {
+    dim3 tg((nc + kPairTile - 1) / kPairTile, (nr + kPairTile - 1) / kPairTile);
+    hipLaunchKernelGGL(k_pair_tiles, tg, dim3(256), 0, ctx->stream, g_ref->d_dev, g_sec->d_dev,
+                       op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
+                       (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
+                       pow_sec, ctx->tile_stats);
+  } else if (interp == S1IFG_INTERP_BILINEAR)
+    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, ctx->stream, g_ref->d_dev,
+                       g_sec->d_dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
+                       (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
+                       pow_sec);
+  else
+    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_SINC>, grd, blk, 0, ctx->stream, g_ref->d_dev,
+                       g_sec->d_dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
+                       (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
+                       pow_sec);
+  S1_LAUNCH_CHECK("k_pair");
   return S1IFG_OK;
 }
 
@@ -275,40 +385,17 @@ int s1ifg_pair(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
                const double *offpoly_sec, const s1ifg_cf32 *ref_deramped,
                const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl, s1ifg_cf32 *ifg_ml,
                float *pow_ref, float *pow_sec) {
-  GridDev gr;
-  DemDev dd;
-  OffPoly op;
-  int rc = common_setup(ctx, g_ref, grid, dem, offpoly_sec, &gr, &dd, &op);
-  if (rc) return rc;
-  if ((rc = check_geom(ctx, g_sec))) return rc;
-  if (!ref_deramped || !sec_deramped || !ifg_ml) return set_error(S1IFG_EINVAL, "null data pointer");
-  if (rl < 1 || cl < 1) return set_error(S1IFG_EINVAL, "looks %d x %d", rl, cl);
-  if (interp != S1IFG_INTERP_BILINEAR && interp != S1IFG_INTERP_SINC)
-    return set_error(S1IFG_EINVAL, "interp %d", interp);
-  int nr = gr.nrow / rl, nc = gr.ncol / cl;
-  if (nr < 1 || nc < 1) return set_error(S1IFG_EINVAL, "grid smaller than one look cell");
-  RowTab *rows;
-  ColTab *cols;
-  if ((rc = grid_tables(ctx, gr, dd, &rows, &cols))) return rc;
-  dim3 blk(64, 4), grd((nc + 63) / 64, (nr + 3) / 4);
-  if (interp == S1IFG_INTERP_SINC && rl * cl <= kPairMaxPix) {
-    dim3 tg((nc + kPairTile - 1) / kPairTile, (nr + kPairTile - 1) / kPairTile);
-    hipLaunchKernelGGL(k_pair_tiles, tg, dim3(256), 0, ctx->stream, g_ref->d_dev, g_sec->d_dev,
-                       op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
-                       (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
-                       pow_sec, ctx->tile_stats);
-  } else if (interp == S1IFG_INTERP_BILINEAR)
-    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, ctx->stream, g_ref->d_dev,
-                       g_sec->d_dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
-                       (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
-                       pow_sec);
-  else
-    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_SINC>, grd, blk, 0, ctx->stream, g_ref->d_dev,
-                       g_sec->d_dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
-                       (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
-                       pow_sec);
-  S1_LAUNCH_CHECK("k_pair");
-  return S1IFG_OK;
+  return pair_impl(ctx, g_ref, g_sec, grid, dem, dh, offpoly_sec, ref_deramped, sec_deramped,
+                   interp, rl, cl, ifg_ml, pow_ref, pow_sec, false);
+}
+
+int s1ifg_pair_noalloc(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
+                       const s1ifg_grid *grid, const s1ifg_dem *dem, const float *dh,
+                       const double *offpoly_sec, const s1ifg_cf32 *ref_deramped,
+                       const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl,
+                       s1ifg_cf32 *ifg_ml, float *pow_ref, float *pow_sec) {
+  return pair_impl(ctx, g_ref, g_sec, grid, dem, dh, offpoly_sec, ref_deramped, sec_deramped,
+                   interp, rl, cl, ifg_ml, pow_ref, pow_sec, true);
 }
 
 }  // extern "C"
