@@ -250,7 +250,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_pair_bwd(
     const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl, int cl, int nr_ml,
     int nc_ml, const float2 *__restrict__ g_ifg, const float *__restrict__ g_pr,
     const float *__restrict__ g_ps, double *__restrict__ partial, float *__restrict__ g_dh) {
-  const GeomDev &Gr = *gpr, &Gs = *gps;
+  const GeomDev &Gr = const_ref(gpr), &Gs = const_ref(gps);
   const OffPoly none = {{0, 0, 0, 0, 0, 0}, 0};
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double gp[6] = {0, 0, 0, 0, 0, 0};
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_bwd_flags(
   const int r0 = ry * kBwdRegion * kBwdRows, c0 = cx * kBwdRegion * kBwdCols;
   const int r1 = min(r0 + kBwdRegion * kBwdRows, nr_ml * rl) - 1;
   const int c1 = min(c0 + kBwdRegion * kBwdCols, nc_ml * cl) - 1;
-  const bool o = tile_outside<S1IFG_INTERP_SINC>(b ? *gps : *gpr, b ? ops : none, dem, rows, cols,
+  const bool o = tile_outside<S1IFG_INTERP_SINC>(b ? const_ref(gps) : const_ref(gpr), b ? ops : none, dem, rows, cols,
                                                  dh, gr.ncol, r0, r1, c0, c1);
   if ((threadIdx.x & 63) == 0) flags[w] = o;
 }
@@ -477,7 +477,8 @@ __global__ __launch_bounds__(kBwdThreads) void k_pair_bwd_tiles(
                               gI.x * z1.y - gI.y * z1.x + g1 * z0.y);
     float2 Gsec = make_float2(gI.x * z0.x - gI.y * z0.y + g2 * z1.x,
                               gI.x * z0.y + gI.y * z0.x + g2 * z1.y);
-    double gh = burst_chain(*gpr, none, b[0], Gref, nullptr) + burst_chain(*gps, ops, b[1], Gsec, gp);
+    double gh = burst_chain(const_ref(gpr), none, b[0], Gref, nullptr) +
+                burst_chain(const_ref(gps), ops, b[1], Gsec, gp);
     if (g_dh) g_dh[(size_t)r * gr.ncol + c] = (float)gh;
   }
   // block sum of the 6 partials: wavefront reductions (fixed order), then

@@ -158,6 +158,7 @@ __global__ void kb_reset(unsigned long long *ctr, int n) {
 
 // Sinc pair kernel with LDS-staged windows over 16 x 16-cell tiles of every
 // pair (rl * cl <= kPairMaxPix), one block per tile in tile order.
+template <int NP, int CL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_pair_tiles(
     const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, int rl, int cl) {
   __shared__ PairLds L;
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const long lt = t - off[p];
   const int tiles_x = (d.nc_ml + kPairTile - 1) / kPairTile;
   const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
-  pair_tile_sinc(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
+  pair_tile_sinc<NP, CL>(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
                  d.sec_d, rl, cl, ty, tx, d.nr_ml, d.nc_ml, d.ifg, d.pr, d.ps, L);
 }
 
@@ -177,6 +178,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // (cache reuse) whatever the grid size; the next tile is fetched while the
 // current one runs.  (A separate kernel: one inlined copy of the tile code
 // per kernel keeps either within the register budget.)
+template <int NP, int CL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_pair_tiles_q(
     const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, int rl, int cl,
     unsigned long long *ctr) {
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const long lt = t - off[p];
     const int tiles_x = (d.nc_ml + kPairTile - 1) / kPairTile;
     const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
-    pair_tile_sinc(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
+    pair_tile_sinc<NP, CL>(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
                    d.sec_d, rl, cl, ty, tx, d.nr_ml, d.nc_ml, d.ifg, d.pr, d.ps, L);
     if (threadIdx.x == 0) slot[cur ^ 1] = nxt;
     cur ^= 1;
@@ -392,12 +394,14 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
   if (long t = total(kStPair)) {
     dim3 blk(o.pair_block_x, 256 / o.pair_block_x);
     dim3 grd(grid_of(t, 1, ncu, o.pair_blocks_per_cu));
+    // the 1 x 4 looks of the workloads with the look shape compiled in
+    const bool l14 = o.rl == 1 && o.cl == 4;
     if (use_pair_tiles(o) && (long)grd.x >= t)
-      hipLaunchKernelGGL(kb_pair_tiles, grd, dim3(256), 0, s, pairs, off[kStPair], n, o.rl,
-                         o.cl);
+      hipLaunchKernelGGL(l14 ? kb_pair_tiles<4, 4> : kb_pair_tiles<0, 0>, grd, dim3(256), 0, s,
+                         pairs, off[kStPair], n, o.rl, o.cl);
     else if (use_pair_tiles(o))
-      hipLaunchKernelGGL(kb_pair_tiles_q, grd, dim3(256), 0, s, pairs, off[kStPair], n, o.rl,
-                         o.cl, b->d_ctr + kStPair);
+      hipLaunchKernelGGL(l14 ? kb_pair_tiles_q<4, 4> : kb_pair_tiles_q<0, 0>, grd, dim3(256), 0,
+                         s, pairs, off[kStPair], n, o.rl, o.cl, b->d_ctr + kStPair);
     else if (o.interp == S1IFG_INTERP_BILINEAR)
       hipLaunchKernelGGL(kb_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, s, pairs, off[kStPair],
                          n, o.rl, o.cl);

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void k_deramp(GeomDev g, const float2 *in, flo
 __global__ void k_geo2rdr(const GeomDev *__restrict__ gp, OffPoly op, GridDev gr, DemDev dem, const RowTab *rows,
                           const ColTab *cols, const float *dh, double *rgoff, double *azoff,
                           double *rng) {
-  const GeomDev &g = *gp;
+  const GeomDev &g = const_ref(gp);
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   int r = blockIdx.y * blockDim.y + threadIdx.y;
   if (r >= gr.nrow || c >= gr.ncol) return;
@@ -67,7 +67,7 @@ template <int INTERP>
 __global__ void k_geocode(const GeomDev *__restrict__ gp, OffPoly op, GridDev gr, DemDev dem, const RowTab *rows,
                           const ColTab *cols, const float *dh, const float2 *__restrict__ z,
                           float2 *__restrict__ out) {
-  const GeomDev &g = *gp;
+  const GeomDev &g = const_ref(gp);
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   int r = blockIdx.y * blockDim.y + threadIdx.y;
   if (r >= gr.nrow || c >= gr.ncol) return;
@@ -88,12 +88,13 @@ __global__ void k_pair(const GeomDev *__restrict__ gpr, const GeomDev *__restric
   int cm = blockIdx.x * blockDim.x + threadIdx.x;
   int rm = blockIdx.y * blockDim.y + threadIdx.y;
   if (rm >= nr_ml || cm >= nc_ml) return;
-  pair_ml_pixel<INTERP>(*gpr, *gps, ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, rm, cm, nc_ml,
+  pair_ml_pixel<INTERP>(const_ref(gpr), const_ref(gps), ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, rm, cm, nc_ml,
                         ifg, pr, ps);
 }
 
 // Sinc pair kernel with LDS-staged burst windows: one 16 x 16 tile of look
 // cells per block (rl * cl <= kPairMaxPix).
+template <int NP, int CL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pair_tiles(
     const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr,
     DemDev dem, const RowTab *rows, const ColTab *cols, const float *dh,
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     int nc_ml, float2 *__restrict__ ifg, float *__restrict__ pr, float *__restrict__ ps,
     unsigned long long *stats) {
   __shared__ PairLds L;
-  pair_tile_sinc(*gpr, *gps, ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, blockIdx.y,
+  pair_tile_sinc<NP, CL>(const_ref(gpr), const_ref(gps), ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, blockIdx.y,
                  blockIdx.x, nr_ml, nc_ml, ifg, pr, ps, L, stats);
 }
 
@@ -272,8 +273,10 @@ int pair_impl(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
   dim3 blk(64, 4), grd((nc + 63) / 64, (nr + 3) / 4);
   if (interp == S1IFG_INTERP_SINC && rl * cl <= kPairMaxPix) {
     dim3 tg((nc + kPairTile - 1) / kPairTile, (nr + kPairTile - 1) / kPairTile);
-    hipLaunchKernelGGL(k_pair_tiles, tg, dim3(256), 0, ctx->stream, g_ref->d_dev, g_sec->d_dev,
-                       op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
+    // the 1 x 4 looks of the workloads with the look shape compiled in
+    auto kern = rl == 1 && cl == 4 ? k_pair_tiles<4, 4> : k_pair_tiles<0, 0>;
+    hipLaunchKernelGGL(kern, tg, dim3(256), 0, ctx->stream, g_ref->d_dev, g_sec->d_dev, op, gr,
+                       dd, rows, cols, dh, (const float2 *)ref_deramped,
                        (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
                        pow_sec, ctx->tile_stats);
   } else if (interp == S1IFG_INTERP_BILINEAR)

@@ -81,6 +81,16 @@ struct OffPoly {
   int on;
 };
 
+// A read-only parameter block in device memory (a kernel's GeomDev, for
+// instance), read through the constant address space: the wavefront-uniform
+// fields then come in by scalar loads into SGPRs instead of one vector load
+// per lane.
+template <typename T>
+__device__ __forceinline__ const T &const_ref(const T *p) {
+  typedef const __attribute__((address_space(4))) T *CP;
+  return *(const T *)(CP)p;
+}
+
 }  // namespace s1ifg
 
 struct s1ifg_ctx {
