@@ -176,6 +176,41 @@ __device__ __forceinline__ int zero_doppler(const GeomDev &g, const double X[3],
   return 52;
 }
 
+// Orbit state at a converged zero-Doppler time: t, x(t), v(t), a(t).
+struct ZdState {
+  double t, x[3], v[3], a[3];
+};
+
+// zero_doppler that also returns the orbit state at the root (for
+// zero_doppler_near on nearby points).
+__device__ __forceinline__ void zero_doppler_state(const GeomDev &g, const double X[3], double &t,
+                                                   double dr[3], ZdState &s) {
+  zero_doppler(g, X, t, dr, s.v, s.a);
+  s.t = t;
+  s.x[0] = X[0] - dr[0];
+  s.x[1] = X[1] - dr[1];
+  s.x[2] = X[2] - dr[2];
+}
+
+// Zero-Doppler time of X from the root state s of a nearby point (a pixel
+// of the same look cell): one Newton step from s.t with the exact
+// derivative, the orbit advanced by the second-order Taylor step of
+// zero_doppler.  Newton's error after the step is f''/(2 f') dt^2; the step
+// is accepted when dt^2 <= g.zd_warm_dt2, the bound (set up with the orbit)
+// under which that error is below 5e-10 s, a tenth of the reference's
+// convergence tolerance.  Longer steps take the full iteration.
+__device__ __forceinline__ void zero_doppler_near(const GeomDev &g, const double X[3],
+                                                  const ZdState &s, double &t, double dr[3]) {
+  double d[3], dt;
+  zd_step(X, s.x, s.v, s.a, d, dt);
+  if (dt * dt <= g.zd_warm_dt2) {
+    t = s.t + dt;
+    zd_finish(d, s.v, s.a, dt, dr);
+  } else {
+    zero_doppler(g, X, t, dr);
+  }
+}
+
 // Row / column tables of the target grid (FP64): see k_grid_tables.
 struct RowTab { double coslat, sinlat, re, fy; };
 struct ColTab { double coslon, sinlon, fx; };

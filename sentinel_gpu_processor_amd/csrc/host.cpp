@@ -259,6 +259,42 @@ static void orbit_eval_host(const std::vector<double> &coef, const double *t, in
   }
 }
 
+// Longest squared time step dt^2 for which one Newton step of the
+// zero-Doppler equation f(t) = (X - x(t)).v(t) = 0, with the exact
+// derivative f' = -|v|^2 + (X - x).a, from a point within dt of the root
+// leaves an error below 5e-10 s (a tenth of the reference's 5e-9 s
+// tolerance): the step's error is f''/(2 f') dt^2 with
+// |f''| <= 3 |v| |a| + |X - x| |j|, bounded over the burst time +- 3 s and
+// for slant ranges up to rmax + 50 km.  The constant is floored at
+// 1e-3 / s, the value of a curved orbit at 700 km, so that the rule does
+// not depend on the orbit being straight.
+static double warm_step_limit(const std::vector<double> &coef, const double *t, int n,
+                              double t0, double t1, double rmax) {
+  double vmin = INFINITY, vmax = 0, amax = 0, jmax = 0;
+  const int m = 256;
+  for (int i = 0; i <= m; ++i) {
+    const double tq = t0 - 3.0 + (t1 - t0 + 6.0) * i / m;
+    double x[3], v[3], a[3], xa[3], va[3], aa[3], xb[3], vb[3], ab[3];
+    const double hh = 1e-2;
+    orbit_eval_host(coef, t, n, tq, x, v, a);
+    orbit_eval_host(coef, t, n, tq - hh, xa, va, aa);
+    orbit_eval_host(coef, t, n, tq + hh, xb, vb, ab);
+    const double vn = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double an = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    double jn = 0;
+    for (int k = 0; k < 3; ++k) jn += std::pow((ab[k] - aa[k]) / (2 * hh), 2);
+    vmin = std::fmin(vmin, vn);
+    vmax = std::fmax(vmax, vn);
+    amax = std::fmax(amax, an);
+    jmax = std::fmax(jmax, std::sqrt(jn));
+  }
+  const double R = rmax + 50e3;
+  const double den = vmin * vmin - R * amax;
+  if (!(den > 0)) return 0.0;  // no warm steps
+  const double C = std::fmax(2.0 * (3 * vmax * amax + R * jmax) / (2 * den), 1e-3);
+  return 5e-10 / C;
+}
+
 static int closest_sample(const double *rec, int n, double t0) {
   // geo2rdr.cu:48-59 on records (t, t0, c0, c1, c2)
   int idx = 0;
@@ -476,6 +512,8 @@ int s1ifg_geom_create(s1ifg_ctx *ctx, const s1ifg_orbit *orbit, const s1ifg_burs
         r[10 + comp] = 6 * c[3] / (h * h * h);
       }
     }
+  gd.zd_warm_dt2 = warm_step_limit(coef, orbit->t, n, gd.tstart, tend,
+                                    gd.rngstart + N * gd.dmrg);
   gd.inv_dmrg = 1.0 / gd.dmrg;
   gd.k4pi_wvl = 4.0 * kPi / gd.wvl;
   gd.inv_dtaz = b->prf;

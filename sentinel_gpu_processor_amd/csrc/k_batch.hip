@@ -396,12 +396,13 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
     dim3 grd(grid_of(t, 1, ncu, o.pair_blocks_per_cu));
     // the 1 x 4 looks of the workloads with the look shape compiled in
     const bool l14 = o.rl == 1 && o.cl == 4;
+    auto tiles = l14 ? kb_pair_tiles<4, 4> : kb_pair_tiles<0, 0>;
+    auto tiles_q = l14 ? kb_pair_tiles_q<4, 4> : kb_pair_tiles_q<0, 0>;
     if (use_pair_tiles(o) && (long)grd.x >= t)
-      hipLaunchKernelGGL(l14 ? kb_pair_tiles<4, 4> : kb_pair_tiles<0, 0>, grd, dim3(256), 0, s,
-                         pairs, off[kStPair], n, o.rl, o.cl);
+      hipLaunchKernelGGL(tiles, grd, dim3(256), 0, s, pairs, off[kStPair], n, o.rl, o.cl);
     else if (use_pair_tiles(o))
-      hipLaunchKernelGGL(l14 ? kb_pair_tiles_q<4, 4> : kb_pair_tiles_q<0, 0>, grd, dim3(256), 0,
-                         s, pairs, off[kStPair], n, o.rl, o.cl, b->d_ctr + kStPair);
+      hipLaunchKernelGGL(tiles_q, grd, dim3(256), 0, s, pairs, off[kStPair], n, o.rl, o.cl,
+                         b->d_ctr + kStPair);
     else if (o.interp == S1IFG_INTERP_BILINEAR)
       hipLaunchKernelGGL(kb_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, s, pairs, off[kStPair],
                          n, o.rl, o.cl);

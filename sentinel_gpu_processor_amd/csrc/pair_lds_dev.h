@@ -64,15 +64,6 @@ __device__ __forceinline__ void stage_window(const float2 *__restrict__ z, int n
   }
 }
 
-// sinc_interp on the burst in global memory, for tiles whose tap window
-// exceeds the LDS budget (a few percent of them): one out-of-line copy
-// shared by every call site keeps the kernel's code small.
-__device__ __attribute__((noinline)) float2 sinc_interp_global(const float2 *__restrict__ z,
-                                                               int ns, int ia, int ir, float fr,
-                                                               float fa, float sp, float cp) {
-  return sinc_interp(z + (size_t)(ia - 3) * ns + (ir - 3), ns, fr, fa, sp, cp);
-}
-
 // Integer wavefront min / max with the result in every lane.
 __device__ __forceinline__ int wave_min_i(int v) {
   return wave_reduce(v, [](int a, int b) { return min(a, b); });
@@ -89,12 +80,8 @@ __device__ __forceinline__ int wave_max_i(int v) {
 enum { kStatTiles, kStatSkipped, kStatStaged, kStatGlobal, kStatEmpty, kStatCount };
 
 // One tile (tile_y, tile_x) of look cells.  All 256 threads of the block
-// call it (block barriers).  NP = rl * cl and CL = cl when they are known at
-// compile time (the 1 x 4 looks of the workloads), else 0 and the runtime
-// rl_, cl_ apply; rl * cl <= kPairMaxPix.  The two bursts go through one
-// loop body (not unrolled), so only one burst's parameters are live at a
-// time and the kernel's code stays small.  A block that runs another tile
-// afterwards must pass a block barrier first (L is reused).
+// call it (block barriers); rl * cl <= kPairMaxPix.  A block that runs
+// another tile afterwards must pass a block barrier first (L is reused).
 template <int NP, int CL>
 __device__ __forceinline__ void pair_tile_sinc(
     const GeomDev &G0, const GeomDev &G1, const OffPoly &op1, const GridDev &gr,
@@ -102,15 +89,14 @@ __device__ __forceinline__ void pair_tile_sinc(
     const float2 *__restrict__ z0, const float2 *__restrict__ z1, int rl_, int cl_, int tile_y,
     int tile_x, int nr_ml, int nc_ml, float2 *ifg, float *pr, float *ps, PairLds &L,
     unsigned long long *stats = nullptr) {
-  static_assert(NP == 0 || (CL > 0 && NP % CL == 0 && NP <= kPairMaxPix), "look shape");
   const int cl = CL ? CL : cl_;
   const int rl = NP ? NP / CL : rl_;
-  const int np = NP ? NP : rl * cl;
   const OffPoly none = {{0, 0, 0, 0, 0, 0}, 0};
   const int tid = threadIdx.x;
   const int cm = tile_x * kPairTile + (tid & (kPairTile - 1));
   const int rm = tile_y * kPairTile + (tid / kPairTile);
   const bool active = cm < nc_ml && rm < nr_ml;
+  const int np = NP ? NP : rl * cl;
   if (tid < 8) L.box[tid >> 2][tid & 3] = (tid & 3) < 2 ? INT_MAX : INT_MIN;
   // wavefront b < 2: is the whole tile outside burst b?
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -139,43 +125,40 @@ __device__ __forceinline__ void pair_tile_sinc(
     }
     return;
   }
-  // heights of the cell's pixels (rows outer, columns inner), for both bursts
+  // Burst 0: geometry of the cell's pixels (rows outer, columns inner),
+  // their tap window staged in LDS, interpolation.  The pixel heights are
+  // kept for burst 1.
   double hp[kPairMaxPix];
-#pragma unroll
-  for (int p = 0; p < kPairMaxPix; ++p) {
-    hp[p] = 0.0;
-    if (p < np && active) {
-      const int i = p / cl, j = p - i * cl;
-      const int r = rm * rl + i, c = cm * cl + j;
-      hp[p] = dem_height(dem, rows[r].fy, cols[c].fx);
-      if (dh) hp[p] += (double)dh[(size_t)r * gr.ncol + c];
-    }
-  }
-  // Per burst: the FP64 geometry and phase of the cell's pixels, their tap
-  // window staged in LDS (burst 1's once every thread is done with burst
-  // 0's), the interpolation.
-  float2 v0[kPairMaxPix], v1[kPairMaxPix];
-#pragma unroll 1
-  for (int b = 0; b < 2; ++b) {
-    const GeomDev &G = b ? G1 : G0;
-    const OffPoly op = b ? op1 : none;
-    const float2 *__restrict__ z = b ? z1 : z0;
-    const bool outb = b ? out1 : out0;
+  float2 v0[kPairMaxPix];
+  {
     int ir[kPairMaxPix], ia[kPairMaxPix];
     float fr[kPairMaxPix], fa[kPairMaxPix], sn[kPairMaxPix], cs[kPairMaxPix];
     unsigned vmask = 0;
     int lo_r = INT_MAX, lo_a = INT_MAX, hi_r = INT_MIN, hi_a = INT_MIN;
+    ZdState zs;  // pixel 0's root: the cell's other pixels start from it
 #pragma unroll
     for (int p = 0; p < kPairMaxPix; ++p) {
       ir[p] = ia[p] = 0;
       fr[p] = fa[p] = sn[p] = cs[p] = 0.f;
-      if (p < np && active && !outb) {
+      hp[p] = 0.0;
+      if (p < np && active) {
         const int i = p / cl, j = p - i * cl;
+        const int r = rm * rl + i, c = cm * cl + j;
+        const RowTab rt = rows[r];
+        const ColTab ct = cols[c];
+        hp[p] = dem_height(dem, rt.fy, ct.fx);
+        if (dh) hp[p] += (double)dh[(size_t)r * gr.ncol + c];
         double X[3];
-        target_xyz(rows[rm * rl + i], cols[cm * cl + j], hp[p], X);
-        double rg, az, rng;
+        target_xyz(rt, ct, hp[p], X);
+        double t, dr[3], rng, rg, az, drg, daz;
+        if (p == 0)
+          zero_doppler_state(G0, X, t, dr, zs);
+        else
+          zero_doppler_near(G0, X, zs, t, dr);
+        sample_position(G0, none, t, dr, rng, rg, az, drg, daz);
         float s, cc;
-        if (geocode_geom<S1IFG_INTERP_SINC>(G, op, X, rg, az, rng, s, cc)) {
+        if (!out0 && sample_valid<S1IFG_INTERP_SINC>(G0, rg, az)) {
+          reramp_phase(G0, rng, drg, daz, t, s, cc);
           const int r0 = (int)floor(rg), a0 = (int)floor(az);
           ir[p] = r0;
           ia[p] = a0;
@@ -195,41 +178,108 @@ __device__ __forceinline__ void pair_tile_sinc(
       const int w0 = wave_min_i(lo_r), w1 = wave_min_i(lo_a);
       const int w2 = wave_max_i(hi_r), w3 = wave_max_i(hi_a);
       if ((tid & 63) == 0) {
-        atomicMin(&L.box[b][0], w0);
-        atomicMin(&L.box[b][1], w1);
-        atomicMax(&L.box[b][2], w2);
-        atomicMax(&L.box[b][3], w3);
+        atomicMin(&L.box[0][0], w0);
+        atomicMin(&L.box[0][1], w1);
+        atomicMax(&L.box[0][2], w2);
+        atomicMax(&L.box[0][3], w3);
       }
     }
-    __syncthreads();  // the burst's box complete; the previous burst's window read by all
+    __syncthreads();
     // stage the tap window (columns min-3 .. max+4, rows likewise)
-    const int mr = L.box[b][0], ma = L.box[b][1], xr = L.box[b][2], xa = L.box[b][3];
+    const int mr = L.box[0][0], ma = L.box[0][1], xr = L.box[0][2], xa = L.box[0][3];
     const bool any = mr <= xr;
     const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
     const bool staged = any && W * H <= kPairWin;
     const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
-    if (stats && tid == 0 && !outb)
+    if (stats && tid == 0 && !out0)
       atomicAdd(&stats[staged ? kStatStaged : any ? kStatGlobal : kStatEmpty], 1ull);
-    if (staged) stage_window(z, G.samples, wa0, wr0, wc, (int)H, L.win);
+    if (staged) stage_window(z0, G0.samples, wa0, wr0, wc, (int)H, L.win);
     __syncthreads();
-    float2 val[kPairMaxPix];
 #pragma unroll
     for (int p = 0; p < kPairMaxPix; ++p) {
-      val[p] = make_float2(0.f, 0.f);
+      v0[p] = make_float2(0.f, 0.f);
       if (p < np && (vmask & (1u << p))) {
         if (staged)
-          val[p] = sinc_interp(&L.win[(ia[p] - 3 - wa0) * wc + (ir[p] - 3 - wr0)], wc, fr[p],
-                               fa[p], sn[p], cs[p]);
+          v0[p] = sinc_interp(&L.win[(ia[p] - 3 - wa0) * wc + (ir[p] - 3 - wr0)], wc, fr[p],
+                              fa[p], sn[p], cs[p]);
         else
-          val[p] = sinc_interp_global(z, G.samples, ia[p], ir[p], fr[p], fa[p], sn[p], cs[p]);
+          v0[p] = sinc_interp(z0 + (size_t)(ia[p] - 3) * G0.samples + (ir[p] - 3), G0.samples,
+                              fr[p], fa[p], sn[p], cs[p]);
       }
     }
-    if (b == 0) {
+  }
+  // Burst 1: geometry, its tap window in LDS once every thread is done
+  // with burst 0's, interpolation
+  float2 v1[kPairMaxPix];
+  {
+    int ir[kPairMaxPix], ia[kPairMaxPix];
+    float fr[kPairMaxPix], fa[kPairMaxPix], sn[kPairMaxPix], cs[kPairMaxPix];
+    unsigned vmask = 0;
+    int lo_r = INT_MAX, lo_a = INT_MAX, hi_r = INT_MIN, hi_a = INT_MIN;
+    ZdState zs;  // pixel 0's root: the cell's other pixels start from it
 #pragma unroll
-      for (int p = 0; p < kPairMaxPix; ++p) v0[p] = val[p];
-    } else {
+    for (int p = 0; p < kPairMaxPix; ++p) {
+      ir[p] = ia[p] = 0;
+      fr[p] = fa[p] = sn[p] = cs[p] = 0.f;
+      if (p < np && active && !out1) {
+        const int i = p / cl, j = p - i * cl;
+        double X[3];
+        target_xyz(rows[rm * rl + i], cols[cm * cl + j], hp[p], X);
+        double t, dr[3], rng, rg, az, drg, daz;
+        if (p == 0)
+          zero_doppler_state(G1, X, t, dr, zs);
+        else
+          zero_doppler_near(G1, X, zs, t, dr);
+        sample_position(G1, op1, t, dr, rng, rg, az, drg, daz);
+        float s, cc;
+        if (sample_valid<S1IFG_INTERP_SINC>(G1, rg, az)) {
+          reramp_phase(G1, rng, drg, daz, t, s, cc);
+          const int r0 = (int)floor(rg), a0 = (int)floor(az);
+          ir[p] = r0;
+          ia[p] = a0;
+          fr[p] = (float)(rg - r0);
+          fa[p] = (float)(az - a0);
+          sn[p] = s;
+          cs[p] = cc;
+          vmask |= 1u << p;
+          lo_r = min(lo_r, r0);
+          hi_r = max(hi_r, r0);
+          lo_a = min(lo_a, a0);
+          hi_a = max(hi_a, a0);
+        }
+      }
+    }
+    {
+      const int w0 = wave_min_i(lo_r), w1 = wave_min_i(lo_a);
+      const int w2 = wave_max_i(hi_r), w3 = wave_max_i(hi_a);
+      if ((tid & 63) == 0) {
+        atomicMin(&L.box[1][0], w0);
+        atomicMin(&L.box[1][1], w1);
+        atomicMax(&L.box[1][2], w2);
+        atomicMax(&L.box[1][3], w3);
+      }
+    }
+    __syncthreads();  // burst 0's window read by every thread, burst 1's box complete
+    const int mr = L.box[1][0], ma = L.box[1][1], xr = L.box[1][2], xa = L.box[1][3];
+    const bool any = mr <= xr;
+    const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
+    const bool staged = any && W * H <= kPairWin;
+    const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
+    if (stats && tid == 0 && !out1)
+      atomicAdd(&stats[staged ? kStatStaged : any ? kStatGlobal : kStatEmpty], 1ull);
+    if (staged) stage_window(z1, G1.samples, wa0, wr0, wc, (int)H, L.win);
+    __syncthreads();
 #pragma unroll
-      for (int p = 0; p < kPairMaxPix; ++p) v1[p] = val[p];
+    for (int p = 0; p < kPairMaxPix; ++p) {
+      v1[p] = make_float2(0.f, 0.f);
+      if (p < np && (vmask & (1u << p))) {
+        if (staged)
+          v1[p] = sinc_interp(&L.win[(ia[p] - 3 - wa0) * wc + (ir[p] - 3 - wr0)], wc, fr[p],
+                              fa[p], sn[p], cs[p]);
+        else
+          v1[p] = sinc_interp(z1 + (size_t)(ia[p] - 3) * G1.samples + (ir[p] - 3), G1.samples,
+                              fr[p], fa[p], sn[p], cs[p]);
+      }
     }
   }
   // the look means in the order of pair_ml_pixel

@@ -59,6 +59,7 @@ struct GeomDev {
   double ati;    // azimuth_time_interval (deramp eta)
   double inv_dmrg, inv_dtaz, inv_samples, inv_lines;
   double k4pi_wvl;  // 4 pi / wvl (two-way phase per metre of range)
+  double zd_warm_dt2;  // longest squared warm Newton step accepted (zero_doppler_near)
   int lines, samples;
   int flv, llv, fvs, lvs;
   const double *kt;      // [samples]
