@@ -179,6 +179,16 @@ def run_gpu(args):
         K.coherence(ctx, ifg, pr, ps, win=5, out=coh)
         ev.mark(None)
 
+    def fused_raw(ev):
+        ev.mark("pair_raw")
+        K.pair(ctx, gr, gs, grid, dem_t, dem, ref, sec, args.interp, rl, cl, offpoly=offpoly,
+               out=(ifg, pr, ps), raw=True)
+        ev.mark("goldstein")
+        K.goldstein(ctx, ifg, 0.5, mode=gold_mode, out=filt)
+        ev.mark("coherence")
+        K.coherence(ctx, ifg, pr, ps, win=5, out=coh)
+        ev.mark(None)
+
     holder = {}
 
     def batched(ev):
@@ -186,7 +196,7 @@ def run_gpu(args):
         holder["batch"].run(graph=True)
         ev.mark(None)
 
-    forms = {"plain": plain, "fused": fused, "batched": batched}
+    forms = {"plain": plain, "fused": fused, "fused_raw": fused_raw, "batched": batched}
     wanted = [f for f in args.forms.split(",") if f]
     results = {}
     if "batched" in wanted:

@@ -123,13 +123,13 @@ void s1ifg_ctx_destroy(s1ifg_ctx *ctx);
 int s1ifg_ctx_set_stream(s1ifg_ctx *ctx, void *stream);
 int s1ifg_ctx_sync(s1ifg_ctx *ctx);
 /* Validation hook (no reference counterpart): a DEVICE array of
- * S1IFG_TILE_STATS uint64 counters to which later s1ifg_pair calls with
- * sinc interpolation on this context add the classes of their 16 x 16-cell
- * tiles: [0] tiles, [1] burst-tiles skipped by the tile's corner test,
- * [2] burst-tiles interpolated from the LDS-staged tap window, [3]
- * burst-tiles whose window exceeds the LDS budget (taps read from global
- * memory), [4] burst-tiles not skipped without any valid pixel.  NULL
- * turns counting off (the default).  The caller zeroes the array. */
+ * S1IFG_TILE_STATS uint64 counters to which later s1ifg_pair / s1ifg_pair_raw
+ * calls with sinc interpolation on this context add the classes of their
+ * 16 x 16-cell tiles: [0] tiles, [1] burst-tiles skipped by the tile's
+ * corner test, [2] burst-tiles interpolated from the LDS-staged tap window,
+ * [3] burst-tiles whose window exceeds the LDS budget (taps read from
+ * global memory), [4] burst-tiles not skipped without any valid pixel.
+ * NULL turns counting off (the default).  The caller zeroes the array. */
 #define S1IFG_TILE_STATS 5
 int s1ifg_ctx_set_tile_stats(s1ifg_ctx *ctx, uint64_t *counts);
 /* Workspace currently held by the context, in bytes. */
@@ -253,6 +253,18 @@ int s1ifg_pair(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
                const double *offpoly_sec, const s1ifg_cf32 *ref_deramped,
                const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl,
                s1ifg_cf32 *ifg_ml, float *pow_ref, float *pow_sec);
+
+/* s1ifg_pair on the RAW bursts: each burst sample is deramped (the phase of
+ * deramp_burst, geo2rdr.cu:138-158, in FP64, with the hardware sine and
+ * cosine) as the kernel reads it, so no deramped copy is written or read
+ * back.  Replaces the deramp_burst -> reproject -> crossmul chain of
+ * geo2rdr.cu:138-248 and crossmul_daemon.cu:921-975 for one pair in one
+ * pass.  Same arguments and outputs as s1ifg_pair, the bursts raw. */
+int s1ifg_pair_raw(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
+                   const s1ifg_grid *grid, const s1ifg_dem *dem, const float *dh,
+                   const double *offpoly_sec, const s1ifg_cf32 *ref_slc, const s1ifg_cf32 *sec_slc,
+                   int interp, int rl, int cl, s1ifg_cf32 *ifg_ml, float *pow_ref,
+                   float *pow_sec);
 
 /* s1ifg_pair without any device allocation: the same kernels and results,
  * with the grid tables (the call's only scratch memory) in a fixed 2 MiB

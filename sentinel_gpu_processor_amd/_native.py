@@ -140,6 +140,8 @@ SIGNATURES = {
     "s1ifg_goldstein": [_P, _P, _I, _I, _I, _F, _I, _P],
     "s1ifg_pair": [_P, _P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P, _P, _I,
                    _I, _I, _P, _P, _P],
+    "s1ifg_pair_raw": [_P, _P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P, _P, _I,
+                       _I, _I, _P, _P, _P],
     "s1ifg_pair_noalloc": [_P, _P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P,
                            _P, _I, _I, _I, _P, _P, _P],
     "s1ifg_pair_backward": [_P, _P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P,

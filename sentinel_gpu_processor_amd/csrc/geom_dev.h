@@ -270,6 +270,27 @@ __device__ __forceinline__ void sincos_phase(double ph, float &s, float &c) {
   c = __builtin_amdgcn_cosf(r);
 }
 
+// The deramp as the fused kernels apply it to the samples they read
+// (deramp_burst's phase, geo2rdr.cu:138-158): z * exp(i phi),
+// phi = -pi kt (eta - etaref)^2, evaluated in FP64 in revolutions
+// (-kt (eta - etaref)^2 / 2, with kth = -kt / 2 given), reduced to
+// [-1/2, 1/2] and rotated with the hardware sine and cosine (as the pixel
+// phase, sincos_phase).
+__device__ __forceinline__ float2 deramp_rot(double eta, double kth, double etaref, float2 a) {
+  const double d = eta - etaref;
+  const double rv = kth * d * d;
+  const float r = (float)(rv - rint(rv));
+  const float s = __builtin_amdgcn_sinf(r), c = __builtin_amdgcn_cosf(r);
+  return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+// Sample (row, col) of a raw burst, deramped (deramp_rot).
+__device__ __forceinline__ float2 raw_tap(const GeomDev &g, const float2 *__restrict__ z, int row,
+                                          int col) {
+  const double eta = -g.lines * g.ati / 2. + row * g.ati;
+  return deramp_rot(eta, -0.5 * g.kt[col], g.etaref[col], z[(size_t)row * g.samples + col]);
+}
+
 // 8-tap Hann-windowed sinc weights for fractional offset f in [0,1):
 // w_k ~ sinc(f - k) * cos^2(pi (f - k)/8), k = -3..4, normalised to sum 1.
 // With sin(pi (f - k)) = (-1)^k sin(pi f), sinc(f - k) is
@@ -414,11 +435,45 @@ __device__ __forceinline__ float2 sinc_interp(const float2 *__restrict__ base, l
   return make_float2(rx * c - ry * s, rx * s + ry * c);
 }
 
+// sinc_interp on a raw burst: each of the 8 x 8 taps deramped as it is read
+// (the columns' kt and etaref read once).
+__device__ __forceinline__ float2 sinc_interp_raw(const GeomDev &g, const float2 *__restrict__ z,
+                                                  int ia, int ir, float fr, float fa, float sp,
+                                                  float cp) {
+  float wr[8], wa[8];
+  const float nr = sinc_weights(fr, wr);
+  const float na = sinc_weights(fa, wa);
+  double kth[8], er[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    kth[k] = -0.5 * g.kt[ir - 3 + k];
+    er[k] = g.etaref[ir - 3 + k];
+  }
+  float rx = 0.f, ry = 0.f;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int row = ia - 3 + a;
+    const double eta = -g.lines * g.ati / 2. + row * g.ati;
+    const float2 *rp = z + (size_t)row * g.samples + (ir - 3);
+    float ax = 0.f, ay = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float2 v = deramp_rot(eta, kth[k], er[k], rp[k]);
+      ax = fmaf(wr[k], v.x, ax);
+      ay = fmaf(wr[k], v.y, ay);
+    }
+    rx = fmaf(wa[a], ax, rx);
+    ry = fmaf(wa[a], ay, ry);
+  }
+  const float sc = nr * na, c = cp * sc, s = sp * sc;
+  return make_float2(rx * c - ry * s, rx * s + ry * c);
+}
+
 // Geocode one target pixel of one burst (reproject, geo2rdr.cu:194-247).
 // Returns the complex value (0 outside the validity window).  rgoff/azoff
 // are the sampling positions (offset polynomial applied), rng the geometric
 // range.
-template <int INTERP>
+template <int INTERP, bool RAW = false>
 __device__ __forceinline__ float2 geocode_pixel(const GeomDev &g, const OffPoly &op,
                                                 const double X[3], const float2 *__restrict__ z,
                                                 double *rg_out = nullptr,
@@ -439,7 +494,18 @@ __device__ __forceinline__ float2 geocode_pixel(const GeomDev &g, const OffPoly 
     int inta = (int)az;
     double fa = az - inta;
     size_t i1 = (size_t)inta * N + intr;
-    float2 b1 = z[i1], b2 = z[i1 + 1], b3 = z[i1 + N], b4 = z[i1 + N + 1];
+    float2 b1, b2, b3, b4;
+    if (RAW) {  // z is the raw burst: its taps deramped as they are read
+      b1 = raw_tap(g, z, inta, intr);
+      b2 = raw_tap(g, z, inta, intr + 1);
+      b3 = raw_tap(g, z, inta + 1, intr);
+      b4 = raw_tap(g, z, inta + 1, intr + 1);
+    } else {
+      b1 = z[i1];
+      b2 = z[i1 + 1];
+      b3 = z[i1 + N];
+      b4 = z[i1 + N + 1];
+    }
     float c1x = (float)(b1.x * (1 - fr) + b2.x * fr);
     float c1y = (float)(b1.y * (1 - fr) + b2.y * fr);
     float c2x = (float)(b3.x * (1 - fr) + b4.x * fr);
@@ -450,6 +516,7 @@ __device__ __forceinline__ float2 geocode_pixel(const GeomDev &g, const OffPoly 
   }
   int intr = (int)floor(rg);
   int inta = (int)floor(az);
+  if (RAW) return sinc_interp_raw(g, z, inta, intr, (float)(rg - intr), (float)(az - inta), sp, cp);
   return sinc_interp(z + (size_t)(inta - 3) * N + (intr - 3), N, (float)(rg - intr),
                      (float)(az - inta), sp, cp);
 }
@@ -652,7 +719,7 @@ __device__ __forceinline__ void grid_table_entry(const GridDev &gr, const DemDev
 // every target pixel of the rl x cl cell, conj(ref)*sec, |ref|^2, |sec|^2,
 // accumulated as column means then a row mean (the order of cpx_col_look /
 // cpx_row_look, sario.cu:241-286).
-template <int INTERP>
+template <int INTERP, bool RAW = false>
 __device__ __forceinline__ void pair_ml_pixel(const GeomDev &gr_, const GeomDev &gs,
                                               const OffPoly &ops, const GridDev &gr,
                                               const DemDev &dem, const RowTab *rows,
@@ -671,8 +738,8 @@ __device__ __forceinline__ void pair_ml_pixel(const GeomDev &gr_, const GeomDev 
       int c = cm * cl + j;
       double X[3];
       pixel_xyz(gr, dem, rows, cols, dh, r, c, X);
-      float2 a = geocode_pixel<INTERP>(gr_, none, X, zr);
-      float2 b = geocode_pixel<INTERP>(gs, ops, X, zs);
+      float2 a = geocode_pixel<INTERP, RAW>(gr_, none, X, zr);
+      float2 b = geocode_pixel<INTERP, RAW>(gs, ops, X, zs);
       sx += a.x * b.x + a.y * b.y;
       sy += a.x * b.y - a.y * b.x;
       spr += a.x * a.x + a.y * a.y;

@@ -79,7 +79,7 @@ __global__ void k_geocode(const GeomDev *__restrict__ gp, OffPoly op, GridDev gr
 // Fused burst pair: one thread per multilooked pixel; the look cell is
 // accumulated as column means then a row mean (the order of
 // cpx_col_look / cpx_row_look, sario.cu:241-286).
-template <int INTERP>
+template <int INTERP, bool RAW>
 __global__ void k_pair(const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr, DemDev dem,
                        const RowTab *rows, const ColTab *cols, const float *dh,
                        const float2 *__restrict__ zr, const float2 *__restrict__ zs, int rl,
@@ -88,13 +88,13 @@ __global__ void k_pair(const GeomDev *__restrict__ gpr, const GeomDev *__restric
   int cm = blockIdx.x * blockDim.x + threadIdx.x;
   int rm = blockIdx.y * blockDim.y + threadIdx.y;
   if (rm >= nr_ml || cm >= nc_ml) return;
-  pair_ml_pixel<INTERP>(const_ref(gpr), const_ref(gps), ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, rm, cm, nc_ml,
+  pair_ml_pixel<INTERP, RAW>(const_ref(gpr), const_ref(gps), ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, rm, cm, nc_ml,
                         ifg, pr, ps);
 }
 
 // Sinc pair kernel with LDS-staged burst windows: one 16 x 16 tile of look
 // cells per block (rl * cl <= kPairMaxPix).
-template <int NP, int CL>
+template <int NP, int CL, bool RAW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pair_tiles(
     const GeomDev *__restrict__ gpr, const GeomDev *__restrict__ gps, OffPoly ops, GridDev gr,
     DemDev dem, const RowTab *rows, const ColTab *cols, const float *dh,
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     int nc_ml, float2 *__restrict__ ifg, float *__restrict__ pr, float *__restrict__ ps,
     unsigned long long *stats) {
   __shared__ PairLds L;
-  pair_tile_sinc<NP, CL>(const_ref(gpr), const_ref(gps), ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, blockIdx.y,
+  pair_tile_sinc<NP, CL, RAW>(const_ref(gpr), const_ref(gps), ops, gr, dem, rows, cols, dh, zr, zs, rl, cl, blockIdx.y,
                  blockIdx.x, nr_ml, nc_ml, ifg, pr, ps, L, stats);
 }
 
@@ -237,12 +237,13 @@ namespace {
 
 // The fused burst pair with its grid tables in the context workspace
 // (static_tables = false), or in the file-scope `scratch` array
-// (static_tables = true: nothing is allocated).
+// (static_tables = true: nothing is allocated); raw: the bursts are the raw
+// (not deramped) ones, deramped as their samples are read.
 int pair_impl(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
               const s1ifg_grid *grid, const s1ifg_dem *dem, const float *dh,
               const double *offpoly_sec, const s1ifg_cf32 *ref_deramped,
               const s1ifg_cf32 *sec_deramped, int interp, int rl, int cl, s1ifg_cf32 *ifg_ml,
-              float *pow_ref, float *pow_sec, bool static_tables) {
+              float *pow_ref, float *pow_sec, bool static_tables, bool raw = false) {
   GridDev gr;
   DemDev dd;
   OffPoly op;
@@ -274,21 +275,21 @@ int pair_impl(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
   if (interp == S1IFG_INTERP_SINC && rl * cl <= kPairMaxPix) {
     dim3 tg((nc + kPairTile - 1) / kPairTile, (nr + kPairTile - 1) / kPairTile);
     // the 1 x 4 looks of the workloads with the look shape compiled in
-    auto kern = rl == 1 && cl == 4 ? k_pair_tiles<4, 4> : k_pair_tiles<0, 0>;
+    const bool l14 = rl == 1 && cl == 4;
+    auto kern = raw ? (l14 ? k_pair_tiles<4, 4, true> : k_pair_tiles<0, 0, true>)
+                    : (l14 ? k_pair_tiles<4, 4, false> : k_pair_tiles<0, 0, false>);
     hipLaunchKernelGGL(kern, tg, dim3(256), 0, ctx->stream, g_ref->d_dev, g_sec->d_dev, op, gr,
                        dd, rows, cols, dh, (const float2 *)ref_deramped,
                        (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
                        pow_sec, ctx->tile_stats);
-  } else if (interp == S1IFG_INTERP_BILINEAR)
-    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_BILINEAR>, grd, blk, 0, ctx->stream, g_ref->d_dev,
-                       g_sec->d_dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
-                       (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
-                       pow_sec);
-  else
-    hipLaunchKernelGGL(k_pair<S1IFG_INTERP_SINC>, grd, blk, 0, ctx->stream, g_ref->d_dev,
-                       g_sec->d_dev, op, gr, dd, rows, cols, dh, (const float2 *)ref_deramped,
-                       (const float2 *)sec_deramped, rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref,
-                       pow_sec);
+  } else {
+    auto kern = interp == S1IFG_INTERP_BILINEAR
+                    ? (raw ? k_pair<S1IFG_INTERP_BILINEAR, true> : k_pair<S1IFG_INTERP_BILINEAR, false>)
+                    : (raw ? k_pair<S1IFG_INTERP_SINC, true> : k_pair<S1IFG_INTERP_SINC, false>);
+    hipLaunchKernelGGL(kern, grd, blk, 0, ctx->stream, g_ref->d_dev, g_sec->d_dev, op, gr, dd,
+                       rows, cols, dh, (const float2 *)ref_deramped, (const float2 *)sec_deramped,
+                       rl, cl, nr, nc, (float2 *)ifg_ml, pow_ref, pow_sec);
+  }
   S1_LAUNCH_CHECK("k_pair");
   return S1IFG_OK;
 }
@@ -390,6 +391,15 @@ int s1ifg_pair(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
                float *pow_ref, float *pow_sec) {
   return pair_impl(ctx, g_ref, g_sec, grid, dem, dh, offpoly_sec, ref_deramped, sec_deramped,
                    interp, rl, cl, ifg_ml, pow_ref, pow_sec, false);
+}
+
+int s1ifg_pair_raw(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,
+                   const s1ifg_grid *grid, const s1ifg_dem *dem, const float *dh,
+                   const double *offpoly_sec, const s1ifg_cf32 *ref_slc, const s1ifg_cf32 *sec_slc,
+                   int interp, int rl, int cl, s1ifg_cf32 *ifg_ml, float *pow_ref,
+                   float *pow_sec) {
+  return pair_impl(ctx, g_ref, g_sec, grid, dem, dh, offpoly_sec, ref_slc, sec_slc, interp, rl,
+                   cl, ifg_ml, pow_ref, pow_sec, false, true);
 }
 
 int s1ifg_pair_noalloc(s1ifg_ctx *ctx, const s1ifg_geom *g_ref, const s1ifg_geom *g_sec,

@@ -64,6 +64,44 @@ __device__ __forceinline__ void stage_window(const float2 *__restrict__ z, int n
   }
 }
 
+// stage_window from a raw burst, deramped as it is staged (deramp_rot):
+// wavefront w stages rows w, w + 4, ...; its lanes take consecutive columns
+// (coalesced rows), so a lane keeps its column's kt / etaref across the rows
+// it stages, and kStage rows' loads are issued before their rotations.
+__device__ __forceinline__ void stage_window_deramp(const float2 *__restrict__ z,
+                                                    const GeomDev &g, int wa0, int wr0, int W,
+                                                    int H, float2 *win) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ns = g.samples;
+  const double eta0 = -g.lines * g.ati / 2.;
+  for (int c = lane; c < W; c += 64) {
+    const int col = wr0 + c;
+    const double kth = -0.5 * g.kt[col], er = g.etaref[col];
+    for (int r0 = wave; r0 < H; r0 += 4 * kStage) {
+      float2 v[kStage];
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) {
+        const int r = r0 + 4 * k;
+        if (r < H) v[k] = z[(size_t)(wa0 + r) * ns + col];
+      }
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) {
+        const int r = r0 + 4 * k;
+        if (r < H) win[r * W + c] = deramp_rot(eta0 + (wa0 + r) * g.ati, kth, er, v[k]);
+      }
+    }
+  }
+}
+
+// sinc_interp_raw out of line, for the rare tiles whose window exceeds the
+// LDS budget (one copy shared by every call site).
+__device__ __attribute__((noinline)) float2 sinc_interp_raw_global(const GeomDev &g,
+                                                                   const float2 *__restrict__ z,
+                                                                   int ia, int ir, float fr,
+                                                                   float fa, float sp, float cp) {
+  return sinc_interp_raw(g, z, ia, ir, fr, fa, sp, cp);
+}
+
 // Integer wavefront min / max with the result in every lane.
 __device__ __forceinline__ int wave_min_i(int v) {
   return wave_reduce(v, [](int a, int b) { return min(a, b); });
@@ -82,7 +120,7 @@ enum { kStatTiles, kStatSkipped, kStatStaged, kStatGlobal, kStatEmpty, kStatCoun
 // One tile (tile_y, tile_x) of look cells.  All 256 threads of the block
 // call it (block barriers); rl * cl <= kPairMaxPix.  A block that runs
 // another tile afterwards must pass a block barrier first (L is reused).
-template <int NP, int CL>
+template <int NP, int CL, bool RAW = false>
 __device__ __forceinline__ void pair_tile_sinc(
     const GeomDev &G0, const GeomDev &G1, const OffPoly &op1, const GridDev &gr,
     const DemDev &dem, const RowTab *rows, const ColTab *cols, const float *dh,
@@ -193,7 +231,8 @@ __device__ __forceinline__ void pair_tile_sinc(
     const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
     if (stats && tid == 0 && !out0)
       atomicAdd(&stats[staged ? kStatStaged : any ? kStatGlobal : kStatEmpty], 1ull);
-    if (staged) stage_window(z0, G0.samples, wa0, wr0, wc, (int)H, L.win);
+    if (staged && RAW) stage_window_deramp(z0, G0, wa0, wr0, wc, (int)H, L.win);
+    if (staged && !RAW) stage_window(z0, G0.samples, wa0, wr0, wc, (int)H, L.win);
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < kPairMaxPix; ++p) {
@@ -202,6 +241,8 @@ __device__ __forceinline__ void pair_tile_sinc(
         if (staged)
           v0[p] = sinc_interp(&L.win[(ia[p] - 3 - wa0) * wc + (ir[p] - 3 - wr0)], wc, fr[p],
                               fa[p], sn[p], cs[p]);
+        else if (RAW)
+          v0[p] = sinc_interp_raw_global(G0, z0, ia[p], ir[p], fr[p], fa[p], sn[p], cs[p]);
         else
           v0[p] = sinc_interp(z0 + (size_t)(ia[p] - 3) * G0.samples + (ir[p] - 3), G0.samples,
                               fr[p], fa[p], sn[p], cs[p]);
@@ -260,6 +301,7 @@ __device__ __forceinline__ void pair_tile_sinc(
       }
     }
     __syncthreads();  // burst 0's window read by every thread, burst 1's box complete
+    // stage the tap window (columns min-3 .. max+4, rows likewise)
     const int mr = L.box[1][0], ma = L.box[1][1], xr = L.box[1][2], xa = L.box[1][3];
     const bool any = mr <= xr;
     const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
@@ -267,7 +309,8 @@ __device__ __forceinline__ void pair_tile_sinc(
     const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
     if (stats && tid == 0 && !out1)
       atomicAdd(&stats[staged ? kStatStaged : any ? kStatGlobal : kStatEmpty], 1ull);
-    if (staged) stage_window(z1, G1.samples, wa0, wr0, wc, (int)H, L.win);
+    if (staged && RAW) stage_window_deramp(z1, G1, wa0, wr0, wc, (int)H, L.win);
+    if (staged && !RAW) stage_window(z1, G1.samples, wa0, wr0, wc, (int)H, L.win);
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < kPairMaxPix; ++p) {
@@ -276,6 +319,8 @@ __device__ __forceinline__ void pair_tile_sinc(
         if (staged)
           v1[p] = sinc_interp(&L.win[(ia[p] - 3 - wa0) * wc + (ir[p] - 3 - wr0)], wc, fr[p],
                               fa[p], sn[p], cs[p]);
+        else if (RAW)
+          v1[p] = sinc_interp_raw_global(G1, z1, ia[p], ir[p], fr[p], fa[p], sn[p], cs[p]);
         else
           v1[p] = sinc_interp(z1 + (size_t)(ia[p] - 3) * G1.samples + (ir[p] - 3), G1.samples,
                               fr[p], fa[p], sn[p], cs[p]);

@@ -198,10 +198,12 @@ def goldstein(ctx, ifg, alpha=0.5, n_win=32, mode="fp32", out=None):
 
 
 def pair(ctx, g_ref, g_sec, grid, dem_t, dem, ref_deramped, sec_deramped, interp="bilinear",
-         rl=1, cl=4, dh=None, offpoly=None, out=None, noalloc=False):
+         rl=1, cl=4, dh=None, offpoly=None, out=None, noalloc=False, raw=False):
     """Fused burst pair -> (ifg_ml, pow_ref, pow_sec) on the look grid.
     noalloc=True calls the entry point that allocates no device memory
-    (grid tables in the library's fixed array; pass `out` too)."""
+    (grid tables in the library's fixed array; pass `out` too); raw=True
+    takes the raw bursts and deramps their samples as they are read
+    (s1ifg_pair_raw)."""
     g = N.make_grid(grid)
     d = _dem_args(dem_t, dem)
     br, bs = g_ref.burst, g_sec.burst
@@ -216,7 +218,10 @@ def pair(ctx, g_ref, g_sec, grid, dem_t, dem, ref_deramped, sec_deramped, interp
                _new(shape, torch.float32, ref_deramped.device))
     ifg, pr, ps = out
     _keep, pp = N.offpoly_arg(offpoly)
-    fn = ctx.lib.s1ifg_pair_noalloc if noalloc else ctx.lib.s1ifg_pair
+    if noalloc and raw:
+        raise ValueError("the allocation-free pair takes deramped bursts")
+    fn = (ctx.lib.s1ifg_pair_noalloc if noalloc else
+          ctx.lib.s1ifg_pair_raw if raw else ctx.lib.s1ifg_pair)
     N.check(fn(ctx.handle, g_ref.handle, g_sec.handle, ctypes.byref(g), ctypes.byref(d),
                N.ptr(dh), pp, N.ptr(ref_deramped), N.ptr(sec_deramped), N.INTERP[interp], rl, cl,
                N.ptr(ifg), N.ptr(pr), N.ptr(ps)))

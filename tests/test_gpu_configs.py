@@ -268,3 +268,23 @@ def test_c2_full_size_physics(ctx):
     # check fails
     ang0 = np.abs(np.angle(blocks(ifg[np.ix_(rs, cs)] * mk)[sel]))
     assert np.quantile(ang0, 0.99) > 0.5
+
+
+def test_c2_raw_pair_matches_deramped(ctx, c2):
+    """The raw-burst pair (samples deramped as staged, and per tap on the
+    tiles read from global memory) over the whole C2 pair against deramp +
+    the pair kernel, and against the oracle on the crop."""
+    from sentinel_gpu_processor_amd import kernels as K
+    sc = c2["sc"]
+    ifg, pr, ps = K.pair(ctx, c2["gr"], c2["gs"], sc["grid"], c2["dem_t"], sc["dem"],
+                         sc["ref_slc"], sc["sec_slc"], "sinc", 1, 4, offpoly=sc["offpoly"],
+                         raw=True)
+    d = (ifg - c2["ifg"]).abs().max().item()
+    assert d <= 2e-5 * c2["ifg"].abs().max().item()
+    assert bool(((pr > 0) == (c2["pr"] > 0)).all())
+    r0, c0, nr, nc = C2_CROP
+    g = crop_grid(sc["grid"], r0, c0, nr, nc)
+    oi, o1, o2, _, _ = oracle_pair_crop(sc, _np(sc["ref_slc"]), _np(sc["sec_slc"]), g, "sinc")
+    m = (o1 > 0) & (o2 > 0) & (np.abs(oi) > 1e-3 * np.abs(oi).max())
+    got = _np(ifg)[r0:r0 + nr, c0 // 4:(c0 + nc) // 4]
+    assert phase_rms(got, oi, m) < 1e-4

@@ -195,3 +195,62 @@ def test_long_misreg_physics(ctx):
             assert ang.max() > 0.5, ang.max()
     valid = m[:, :, None].repeat(4, axis=2).reshape(grid["nrow"], nc * 4)
     assert np.ptp(resid[:, :nc * 4][valid]) > 1.0
+
+
+@pytest.mark.parametrize("interp", ["bilinear", "sinc"])
+@pytest.mark.parametrize("name", ["small_misreg", "small_tops"])
+def test_pair_raw_matches_oracle(ctx, interp, name):
+    """s1ifg_pair_raw (raw bursts, deramped as they are read) against the
+    oracle chain and against deramp + s1ifg_pair."""
+    from sentinel_gpu_processor_amd import kernels as K
+    sc = scene(name)
+    gr, gs = _geoms(ctx, sc)
+    oc = oracle_chain(name, interp)
+    dem = sc["dem"]
+    dem_t = _dev(dem["data"])
+    ref, sec = _dev(sc["ref_slc"]), _dev(sc["sec_slc"])
+    ifg, pr, ps = (_np(t) for t in K.pair(ctx, gr, gs, sc["grid"], dem_t, dem, ref, sec, interp,
+                                          1, 4, offpoly=sc["offpoly"], raw=True))
+    m = (oc["p1"] > 0) & (oc["p2"] > 0) & (np.abs(oc["ifg"]) > 1e-3 * np.abs(oc["ifg"]).max())
+    assert m.mean() > 0.2
+    assert phase_rms(ifg, oc["ifg"], m) < 1e-4
+    assert np.count_nonzero((pr > 0) != (oc["p1"] > 0)) <= 2
+    dref, dsec = K.deramp(ctx, gr, ref), K.deramp(ctx, gs, sec)
+    i2, p2, s2 = (_np(t) for t in K.pair(ctx, gr, gs, sc["grid"], dem_t, dem, dref, dsec, interp,
+                                         1, 4, offpoly=sc["offpoly"]))
+    assert np.max(np.abs(ifg - i2)) <= 2e-5 * np.abs(i2).max()
+    assert np.max(np.abs(pr - p2)) <= 2e-5 * p2.max()
+
+
+@pytest.mark.parametrize("raw", [False, True])
+def test_pair_coarse_grid_global_taps(ctx, raw):
+    """A target grid 8 x 4 times coarser than the bursts' sampling: one
+    wavefront's rows of a tile span a tap window beyond the LDS budget, so
+    the taps are read from global memory (deramped per tap when raw); the
+    products still match the oracle on that grid."""
+    from sentinel_gpu_processor_amd import _native as N
+    from sentinel_gpu_processor_amd import kernels as K
+    from tests.configs import oracle_pair_crop
+    sc = scene("small_misreg")
+    g = dict(sc["grid"], nrow=sc["grid"]["nrow"] // 8, ncol=sc["grid"]["ncol"] // 4,
+             dlat=sc["grid"]["dlat"] * 8, dlon=sc["grid"]["dlon"] * 4)
+    gr, gs = _geoms(ctx, sc)
+    dem = sc["dem"]
+    ref, sec = _dev(sc["ref_slc"]), _dev(sc["sec_slc"])
+    if not raw:
+        ref, sec = K.deramp(ctx, gr, ref), K.deramp(ctx, gs, sec)
+    stats = torch.zeros(N.TILE_STATS, dtype=torch.int64, device="cuda:0")
+    ctx.set_tile_stats(stats)
+    try:
+        out = K.pair(ctx, gr, gs, g, _dev(dem["data"]), dem, ref, sec, "sinc", 1, 4,
+                     offpoly=sc["offpoly"], raw=raw)
+        st = _np(stats).tolist()
+    finally:
+        ctx.set_tile_stats(None)
+    assert st[3] > 0, st
+    ifg, pr, ps = (_np(t) for t in out)
+    oi, o1, o2, _, _ = oracle_pair_crop(sc, sc["ref_slc"], sc["sec_slc"], g, "sinc")
+    m = (o1 > 0) & (o2 > 0) & (np.abs(oi) > 1e-3 * np.abs(oi).max())
+    assert m.mean() > 0.2
+    assert phase_rms(ifg, oi, m) < 1e-4
+    assert np.count_nonzero((pr > 0) != (o1 > 0)) <= 2
