@@ -53,7 +53,7 @@ def parse():
     p.add_argument("--stack-dates", type=int, default=20)
     p.add_argument("--gold-mode", default="auto", choices=["auto", "fp32", "fp16_split"])
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-sample", default="c2crop", choices=["c2crop", "c1"])
+    p.add_argument("--cpu-sample", default="c2", choices=["c2", "c1"])
     p.add_argument("--calibrate-copy", action="store_true",
                    help="(default) time a plain 1 GiB device copy: the measured streaming "
                         "rate, and the reading that scales FETCH_SIZE / WRITE_SIZE counters")
@@ -650,50 +650,70 @@ def calibrate_copy(nbytes=1 << 30, reps=3):
     return gbs
 
 
-def cpu_baseline(sample):
-    """FP64 NumPy oracle chain on a bounded sample (single process): "c2crop"
-    is the C2 pair's generator, DEM and offsets cut to 1500 x 3072 with the
-    benchmark's sinc resampling (~15 s of CPU work); "c1" is the reference's
-    own CPU-runnable case (bilinear)."""
-    import numpy as np
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
-    from oracle import geom as G
-    from oracle import goldstein as GS
-    from oracle import ifg as I
+
+def cpu_baseline(sample):
+    """The oracle's C/OpenMP restatement (oracle/c/s1cpu.c, FP64 geometry,
+    float samples) of the whole pair chain on every host thread OpenMP is
+    given: "c2" is the benchmark's own 1500 x 25000 pair (same generator,
+    DEM, offsets and sinc resampling as the GPU run, ~3-6 s on 8-16
+    threads); "c1" is the reference's CPU-runnable case (bilinear)."""
+    from oracle import cport as C
     from sentinel_gpu_processor_amd import synthetic as S
 
-    if sample == "c2crop":
-        cfg = dict(S.CONFIGS["c2"], samples=3072)
-        interp = "sinc"
-    else:
-        cfg = S.CONFIGS[sample]
-        interp = "bilinear"
+    cfg = S.CONFIGS[sample]
+    interp = "sinc" if sample == "c2" else "bilinear"
     sc = S.make_scene(lines=cfg["lines"], samples=cfg["samples"], dem_shape=cfg["dem"],
                       dem_kind=cfg["dem_kind"], hmax=cfg["hmax"], seed=0, fidelity="fast",
-                      device="cuda:0", misreg=cfg["offset"])
+                      device="cuda:0", misreg=cfg["offset"],
+                      misreg_ramp=(0.2, 0.1) if cfg["offset"] else (0, 0))
     ref = sc["ref_slc"].cpu().numpy()
     sec = sc["sec_slc"].cpu().numpy()
+    grid, dem, offpoly = sc["grid"], sc["dem"], sc["offpoly"]
+    stages = {}
     t0 = time.perf_counter()
-    b = sc["burst_ref"]
-    h = G.target_heights(sc["grid"], sc["dem"])
-    su = G.deramp_setup(sc["orbit_ref"], b)
-    ss = G.deramp_setup(sc["orbit_sec"], b)
-    dr = G.deramp(ref, su).astype(np.complex64)
-    ds = G.deramp(sec, ss).astype(np.complex64)
-    o_r, _ = G.reproject(sc["orbit_ref"], b, su, sc["grid"], h, dr, interp=interp)
-    o_s, _ = G.reproject(sc["orbit_sec"], b, ss, sc["grid"], h, ds, interp=interp,
-                         offpoly=sc["offpoly"])
-    ifg = I.cpx_looks(I.conj_mul(o_r, o_s), 1, 4)
-    p1 = I.cpx_looks(np.abs(o_r) ** 2, 1, 4).real
-    p2 = I.cpx_looks(np.abs(o_s) ** 2, 1, 4).real
-    GS.goldstein(ifg, 32, 0.5)
-    I.coherence_window(ifg, p1, p2, 5)
-    dt = time.perf_counter() - t0
+    t = t0
+
+    def mark(name):
+        nonlocal t
+        now = time.perf_counter()
+        stages[name] = round(now - t, 3)
+        t = now
+
+    kr, er, sr = C.deramp_setup(sc["orbit_ref"], sc["burst_ref"])
+    ks, es, ss = C.deramp_setup(sc["orbit_sec"], sc["burst_sec"])
+    mark("setup")
+    dr = C.deramp(sc["burst_ref"], kr, er, ref)
+    ds = C.deramp(sc["burst_sec"], ks, es, sec)
+    mark("deramp")
+    o_r = C.reproject(sc["orbit_ref"], sc["burst_ref"], sr, grid, dem, dr, interp)
+    o_s = C.reproject(sc["orbit_sec"], sc["burst_sec"], ss, grid, dem, ds, interp, offpoly)
+    mark("geocode")
+    ifg, p1, p2 = C.crossmul_looks(o_r, o_s, 1, 4)
+    mark("crossmul_looks")
+    C.goldstein(ifg, 0.5)
+    mark("goldstein")
+    C.coherence(ifg, p1, p2, 5)
+    mark("coherence")
+    dt = t - t0
     npx = cfg["lines"] * cfg["samples"]
-    return dict(value=npx / dt / 1e6, unit="Mpix/s", cores=1, kind="port",
-                sample=f"{sample}: one {cfg['lines']}x{cfg['samples']} burst pair through the "
-                       f"FP64 NumPy oracle chain ({interp}, 1x4 looks, Goldstein 32/0.5, 5x5 "
-                       f"coherence) in {dt:.1f} s")
+    threads = C.threads()
+    return dict(value=round(npx / dt / 1e6, 2), unit="Mpix/s", cores=threads, kind="port",
+                cpu=cpu_model(), omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                seconds=round(dt, 3), stages_s=stages,
+                sample=f"{sample}: one {cfg['lines']}x{cfg['samples']} burst pair, the whole "
+                       f"chain (deramp set-up and deramp x 2, {interp} geocode x 2, crossmul + "
+                       f"1x4 looks, Goldstein 32/8 alpha 0.5, 5x5 coherence) through the C/OpenMP "
+                       f"restatement of the oracle on {threads} threads, {dt:.2f} s")
 
 
 def main():
@@ -708,7 +728,7 @@ def main():
     balg = b_alg(L, Nr, dem_px, rl, cl)
     sb = stage_bytes(L, Nr, dem_px, rl, cl)
     forms = {}
-    for name in ("plain", "fused", "batched"):
+    for name in ("plain", "fused", "fused_raw", "batched"):
         if name not in res:
             continue
         r = res[name]

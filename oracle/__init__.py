@@ -15,6 +15,11 @@ repository):
                 coherence.py:14-147; sario.py:397-420; interfere.py:16-180
 - ``goldstein`` goldstein.cu:42-408
 - ``sinc``      windowed-sinc interpolator (no reference counterpart)
+- ``cport``     ctypes binding of ``oracle/c/s1cpu.c`` (built by
+                ``make -C oracle all`` into ``oracle/libs1cpu.so``): the same
+                chain in C11 + OpenMP, the CPU timing beside the GPU; its
+                header cites the reference lines per function, and
+                tests/test_oracle_cport.py pins it against the modules above
 
 Parity status per function is listed in each module's header.
 """
