@@ -43,7 +43,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--config", default="c2", choices=["c1", "c2"])
     p.add_argument("--interp", default="sinc", choices=["bilinear", "sinc"])
-    p.add_argument("--forms", default="plain,fused,batched")
+    p.add_argument("--forms", default="plain,fused,fused_raw,batched")
     p.add_argument("--tune-log", action="store_true", help="print every launch shape the autotuner times")
     p.add_argument("--no-iw", action="store_true", help="skip the 27-burst IW pair")
     p.add_argument("--iw-steps", type=int, default=5)
