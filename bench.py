@@ -50,6 +50,7 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true", help="skip the C2 forward+backward")
     p.add_argument("--no-stack", action="store_true", help="skip the 20-date stack")
     p.add_argument("--no-stitch", action="store_true", help="skip the IW subswath stitch")
+    p.add_argument("--no-ingest", action="store_true", help="skip the CInt16 deramp leg")
     p.add_argument("--stack-dates", type=int, default=20)
     p.add_argument("--gold-mode", default="auto", choices=["auto", "fp32", "fp16_split"])
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -220,6 +221,11 @@ def run_gpu(args):
     results["_shape"] = (L, Nr, dem["data"].size, rl, cl)
     if "batch" in holder:
         holder["batch"].close()
+    if not args.no_ingest:
+        try:
+            results["_ingest"] = run_ingest(ctx, gr, ref, dref)
+        except Exception as e:  # reported; the forward numbers stand alone
+            results["_ingest"] = dict(error=repr(e))
     if not args.no_adjoint:
         try:
             results["_adjoint"] = run_adjoint(args, ctx, gr, gs, grid, dem_t, dem, dref, dsec,
@@ -227,6 +233,61 @@ def run_gpu(args):
         except Exception as e:  # reported; the forward numbers stand alone
             results["_adjoint"] = dict(error=repr(e))
     return results
+
+
+def run_ingest(ctx, geom, ref, dref, reps=20):
+    """The C2 reference burst deramped from CInt16 (the measurement format,
+    4 B per sample read, 8 B written) against the complex-float deramp
+    (8 + 8 B), device-resident; and both again with the burst's host-to-
+    device copy from pinned memory in the timed region."""
+    import torch
+
+    from sentinel_gpu_processor_amd import kernels as K
+
+    L, Nr = ref.shape
+    raw = torch.randint(-3000, 3000, (L, Nr, 2), dtype=torch.int16, device=ref.device)
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    ms16 = timed(lambda: K.deramp_ci16(ctx, geom, raw, out=dref), reps)
+    ms32 = timed(lambda: K.deramp(ctx, geom, ref, out=dref), reps)
+    h16 = torch.empty((L, Nr, 2), dtype=torch.int16, pin_memory=True)
+    h16.copy_(raw.cpu())
+    h32 = torch.empty((L, Nr), dtype=torch.complex64, pin_memory=True)
+    h32.copy_(ref.cpu())
+    d16 = torch.empty_like(raw)
+
+    def up16():
+        d16.copy_(h16, non_blocking=True)
+        K.deramp_ci16(ctx, geom, d16, out=dref)
+
+    def up32():
+        dref.copy_(h32, non_blocking=True)
+        K.deramp(ctx, geom, dref, out=dref)
+
+    pc16 = timed(up16, 5)
+    pc32 = timed(up32, 5)
+    n = L * Nr
+    r = dict(samples=n, ci16_ms=round(ms16, 4), ci16_gbs=round(12 * n / (ms16 * 1e-3) / 1e9, 1),
+             ci16_hbm_frac=round(12 * n / (ms16 * 1e-3) / HBM_PEAK, 4),
+             cf32_ms=round(ms32, 4), cf32_gbs=round(16 * n / (ms32 * 1e-3) / 1e9, 1),
+             h2d_ci16_ms=round(pc16, 3), h2d_cf32_ms=round(pc32, 3),
+             h2d_ci16_msamples_s=round(n / (pc16 * 1e-3) / 1e6, 1),
+             h2d_cf32_msamples_s=round(n / (pc32 * 1e-3) / 1e6, 1))
+    print(f"[bench] ingest: CInt16 deramp {ms16:.3f} ms ({r['ci16_gbs']} GB/s), cf32 deramp "
+          f"{ms32:.3f} ms ({r['cf32_gbs']} GB/s); with the pinned H2D copy {pc16:.2f} vs "
+          f"{pc32:.2f} ms", file=sys.stderr, flush=True)
+    return r
 
 
 def run_adjoint(args, ctx, gr, gs, grid, dem_t, dem, dref, dsec, offpoly, rl, cl):
@@ -820,6 +881,7 @@ def main():
         "adjoint": res.get("_adjoint"),
         "stack": stack,
         "stitch": stitch,
+        "ingest": res.get("_ingest"),
         "roofline": roof,
         "cpu_baseline": cpu,
     }

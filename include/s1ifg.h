@@ -165,6 +165,16 @@ int s1ifg_orbit_eval(s1ifg_ctx *ctx, const s1ifg_geom *g, const double *t,
  * out.  Replaces deramp_burst (geo2rdr.cu:138-158). */
 int s1ifg_deramp(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_cf32 *in,
                  s1ifg_cf32 *out);
+/* A8 deramp of a burst in the Sentinel-1 SLC measurement format: in is
+ * interleaved CInt16 (re, im int16 per sample, lines x samples, DEVICE),
+ * out complex float (must not overlap in).  The int16 -> float conversion
+ * is exact, so out equals s1ifg_deramp of the converted burst bit for bit;
+ * it reads 4 bytes per sample instead of 8.  Replaces the CInt16 -> cf32
+ * conversion of the TIFF stream (sentinel_scene.py:164-207, GDAL
+ * ReadAsArray) plus deramp_burst on the streamed burst (geo2rdr.cu:552-564,
+ * 138-158). */
+int s1ifg_deramp_ci16(s1ifg_ctx *ctx, const s1ifg_geom *g, const int16_t *in,
+                      s1ifg_cf32 *out);
 
 /* A5/A9 geo2rdr offsets on the target grid (validation mode: the fused
  * kernels never materialise them).  dh: optional per-pixel DEM error (float,

@@ -129,6 +129,7 @@ SIGNATURES = {
     "s1ifg_geom_vectors": [_P, _P, _P, _I],
     "s1ifg_orbit_eval": [_P, _P, _P, _I64, _P, _P, _P],
     "s1ifg_deramp": [_P, _P, _P, _P],
+    "s1ifg_deramp_ci16": [_P, _P, _P, _P],
     "s1ifg_geo2rdr": [_P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P, _P, _P],
     "s1ifg_geocode": [_P, _P, ctypes.POINTER(Grid), ctypes.POINTER(Dem), _P, _P, _P, _I, _P],
     "s1ifg_valid_rows": [_P, _P, _I, _I, _P],

@@ -688,6 +688,49 @@ __device__ __forceinline__ void deramp_tile(const GeomDev &g, const float2 *in, 
   }
 }
 
+// The same deramp of a burst stored as interleaved CInt16 (re, im int16
+// per sample, the Sentinel-1 SLC measurement format), written as complex
+// float: the int16 -> float conversion is exact, so the result equals
+// deramp_tile of the converted burst bit for bit.  vec: 8-byte loads of two
+// samples and 16-byte stores (even row length, aligned rows); otherwise one
+// sample at a time.  in and out never alias (different element sizes).
+__device__ __forceinline__ void deramp_tile_ci16(const GeomDev &g, const short *__restrict__ in,
+                                                 float2 *__restrict__ out, int row0, int pair0,
+                                                 bool vec) {
+  const int N = g.samples;
+  const int jj = pair0 + (int)threadIdx.x;
+  if (jj >= (N + 1) / 2) return;
+  const int j = 2 * jj;
+  const int nr = min(kDerampRows, g.lines - row0);
+  const double eta0 = -g.lines * g.ati / 2.;
+  if (vec) {
+    const short4 *in4 = (const short4 *)in;
+    float4 *out4 = (float4 *)out;
+    const double kt0 = g.kt[j], kt1 = g.kt[j + 1];
+    const double er0 = g.etaref[j], er1 = g.etaref[j + 1];
+    short4 a[kDerampRows];
+#pragma unroll
+    for (int r = 0; r < kDerampRows; ++r)
+      if (r < nr) a[r] = in4[(((size_t)(row0 + r) * N) >> 1) + jj];
+#pragma unroll
+    for (int r = 0; r < kDerampRows; ++r) {
+      if (r >= nr) break;
+      const double eta = eta0 + (row0 + r) * g.ati;
+      const float2 o0 = deramp_elem(eta, kt0, er0, make_float2(a[r].x, a[r].y));
+      const float2 o1 = deramp_elem(eta, kt1, er1, make_float2(a[r].z, a[r].w));
+      out4[(((size_t)(row0 + r) * N) >> 1) + jj] = make_float4(o0.x, o0.y, o1.x, o1.y);
+    }
+  } else {
+    for (int r = 0; r < nr; ++r) {
+      const double eta = eta0 + (row0 + r) * g.ati;
+      const size_t rowoff = (size_t)(row0 + r) * N;
+      for (int k = j; k < j + 2 && k < N; ++k)
+        out[rowoff + k] = deramp_elem(eta, g.kt[k], g.etaref[k],
+                                      make_float2(in[2 * (rowoff + k)], in[2 * (rowoff + k) + 1]));
+    }
+  }
+}
+
 // Grid tables entry i (llh2xyz trigonometry, orbit.cu:15-35, and DEM
 // sampling coordinates).
 __device__ __forceinline__ void grid_table_entry(const GridDev &gr, const DemDev &dem, int i,

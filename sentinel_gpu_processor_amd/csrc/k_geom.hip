@@ -36,6 +36,12 @@ __global__ __launch_bounds__(256) void k_deramp(GeomDev g, const float2 *in, flo
   deramp_tile(g, in, out, blockIdx.y * kDerampRows, blockIdx.x * blockDim.x, vec != 0);
 }
 
+// The deramp of a CInt16 burst into complex float (deramp_tile_ci16).
+__global__ __launch_bounds__(256) void k_deramp_ci16(GeomDev g, const short *in, float2 *out,
+                                                     int vec) {
+  deramp_tile_ci16(g, in, out, blockIdx.y * kDerampRows, blockIdx.x * blockDim.x, vec != 0);
+}
+
 __global__ void k_geo2rdr(const GeomDev *__restrict__ gp, OffPoly op, GridDev gr, DemDev dem, const RowTab *rows,
                           const ColTab *cols, const float *dh, double *rgoff, double *azoff,
                           double *rng) {
@@ -324,6 +330,26 @@ int s1ifg_deramp(s1ifg_ctx *ctx, const s1ifg_geom *g, const s1ifg_cf32 *in, s1if
   hipLaunchKernelGGL(k_deramp, dim3(bx, (gd.lines + kDerampRows - 1) / kDerampRows), dim3(256),
                      0, ctx->stream, gd, (const float2 *)in, (float2 *)out, vec);
   S1_LAUNCH_CHECK("k_deramp");
+  return S1IFG_OK;
+}
+
+int s1ifg_deramp_ci16(s1ifg_ctx *ctx, const s1ifg_geom *g, const int16_t *in, s1ifg_cf32 *out) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if ((rc = check_geom(ctx, g))) return rc;
+  if (!in || !out) return set_error(S1IFG_EINVAL, "null burst pointer");
+  const GeomDev &gd = g->dev;
+  const uintptr_t ib = (uintptr_t)in, ob = (uintptr_t)out;
+  const uintptr_t ie = ib + (size_t)gd.lines * gd.samples * 4;
+  const uintptr_t oe = ob + (size_t)gd.lines * gd.samples * 8;
+  if (ib < oe && ob < ie) return set_error(S1IFG_EINVAL, "CInt16 input overlaps the output");
+  int pairs = (gd.samples + 1) / 2;
+  int bx = (pairs + 255) / 256;
+  // 8-byte loads / 16-byte stores: even row length and aligned rows
+  int vec = (gd.samples % 2 == 0) && (ib % 8 == 0) && (ob % 16 == 0);
+  hipLaunchKernelGGL(k_deramp_ci16, dim3(bx, (gd.lines + kDerampRows - 1) / kDerampRows),
+                     dim3(256), 0, ctx->stream, gd, (const short *)in, (float2 *)out, vec);
+  S1_LAUNCH_CHECK("k_deramp_ci16");
   return S1IFG_OK;
 }
 

@@ -118,7 +118,8 @@ def geocode_bursts(orbit, bursts, burst_data, dem, rsc, slcoutfile, hmin, hmax,
     """Geocode every burst and write ``<slcoutfile>_burst_<k>.gslc``.
 
     orbit: (t, pos, vel); bursts: list of s1ifg burst dicts; burst_data:
-    iterable of complex64 (lines x samples) arrays in burst order; dem: 2-D
+    iterable of complex64 (lines x samples) arrays, or int16 (lines x samples
+    x 2) CInt16 arrays as stored in the measurement TIFF, in burst order; dem: 2-D
     int16 / float32 array on the grid ``rsc`` (GeoCoordinates).  Returns the
     list of files written (bursts outside the DEM or all-zero are skipped,
     as geo2rdr.cu:583-584, 632-633).
@@ -143,8 +144,16 @@ def geocode_bursts(orbit, bursts, burst_data, dem, rsc, slcoutfile, hmin, hmax,
                 "dlat": rsc.dlat, "dlon": rsc.dlon}
         demd = dict(grid)
         dem_t = K.from_numpy(sub, dev)
-        z = K.from_numpy(np.asarray(data, np.complex64).reshape(b["lines"], b["samples"]), dev)
-        K.deramp(ctx, geom, z, out=z)
+        arr = np.asarray(data)
+        if arr.dtype == np.int16:
+            # CInt16 measurement samples: converted inside the deramp
+            raw = K.from_numpy(arr.reshape(b["lines"], b["samples"], 2), dev)
+            z = K.deramp_ci16(ctx, geom, raw)
+            del raw
+        else:
+            z = K.from_numpy(arr.astype(np.complex64, copy=False).reshape(b["lines"],
+                                                                        b["samples"]), dev)
+            K.deramp(ctx, geom, z, out=z)
         out = K.geocode(ctx, geom, grid, dem_t, demd, z, interp)
         first, last = K.valid_rows(ctx, out)
         if first == last:
@@ -157,10 +166,7 @@ def geocode_bursts(orbit, bursts, burst_data, dem, rsc, slcoutfile, hmin, hmax,
     return written
 
 
-def geo2rdr_from_db(dbname, slcoutfile, slcinfile="", stream=None, interp="bilinear"):
-    """The ``geo2rdr db slcout [slcin] [--stdin]`` workflow
-    (geo2rdr.cu:316-734): parameters from the database, burst data from
-    ``stream`` (a binary file object, like stdin) or from the SLC file."""
+def _db_inputs(dbname):
     sw, per = bursts_from_db(dbname)
     root = os.path.dirname(os.path.abspath(dbname))
     res = lambda p: p if os.path.isabs(p) else os.path.join(root, p)
@@ -171,6 +177,14 @@ def geo2rdr_from_db(dbname, slcoutfile, slcinfile="", stream=None, interp="bilin
     dem = np.memmap(res(sw["demfile"]), dtype=np.int16, mode="r",
                     shape=(int(rsc.nlat), int(rsc.nlon)))
     bursts = [burst_dict(sw, b, fm, dc) for b in per]
+    return sw, res, orbit, rsc, dem, bursts
+
+
+def geo2rdr_from_db(dbname, slcoutfile, slcinfile="", stream=None, interp="bilinear"):
+    """The ``geo2rdr db slcout [slcin] [--stdin]`` workflow
+    (geo2rdr.cu:316-734): parameters from the database, burst data from
+    ``stream`` (a binary file object, like stdin) or from the SLC file."""
+    sw, res, orbit, rsc, dem, bursts = _db_inputs(dbname)
     n = sw["lines"] * sw["samples"]
 
     def data():
@@ -188,4 +202,19 @@ def geo2rdr_from_db(dbname, slcoutfile, slcinfile="", stream=None, interp="bilin
                 f.close()
 
     return geocode_bursts(orbit, bursts, data(), dem, rsc, slcoutfile, sw["hmin"], sw["hmax"],
+                          interp)
+
+
+def geo2rdr_from_tiff(dbname, slcoutfile, zip_path, tiff_name=None, interp="bilinear"):
+    """The scene step that streams a measurement TIFF into ``geo2rdr db
+    slcout --stdin`` (sentinel_scene.py:164-207, 419-437), in one process:
+    parameters from the database, bursts read from the TIFF (``tiff_name``
+    inside the zip ``zip_path``, or ``zip_path`` itself when ``tiff_name`` is
+    None) as CInt16 and converted on the GPU in the deramp.  Raises
+    RuntimeError when the TIFF's size disagrees with the database, as the
+    reference does.  Returns the .gslc files written."""
+    from .tiff import stream_tiff_from_zip
+    sw, _, orbit, rsc, dem, bursts = _db_inputs(dbname)
+    data = stream_tiff_from_zip(zip_path, tiff_name, sw["samples"], sw["lines"], len(bursts))
+    return geocode_bursts(orbit, bursts, data, dem, rsc, slcoutfile, sw["hmin"], sw["hmax"],
                           interp)

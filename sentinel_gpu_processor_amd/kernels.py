@@ -35,6 +35,20 @@ def deramp(ctx, geom, slc, out=None):
     return out
 
 
+def deramp_ci16(ctx, geom, raw, out=None):
+    """TOPS deramp of one burst given as interleaved CInt16 (an int16 tensor
+    of shape (lines, samples, 2): re, im), written as complex64; the
+    Sentinel-1 measurement format read without a float copy
+    (sentinel_scene.py:164-207 + geo2rdr.cu:138-158)."""
+    b = geom.burst
+    _need(raw, torch.int16, (b["lines"], b["samples"], 2), "CInt16 burst")
+    if out is None:
+        out = torch.empty((b["lines"], b["samples"]), dtype=torch.complex64, device=raw.device)
+    _need(out, torch.complex64, (b["lines"], b["samples"]), "deramped burst")
+    N.check(ctx.lib.s1ifg_deramp_ci16(ctx.handle, geom.handle, N.ptr(raw), N.ptr(out)))
+    return out
+
+
 def _dem_args(dem_t, dem):
     return N.make_dem(dem_t, dem)
 

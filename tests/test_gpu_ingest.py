@@ -169,3 +169,102 @@ def test_geo2rdr_from_db_short_stream(tmp_path):
     with pytest.raises(EOFError):
         GR.geo2rdr_from_db(str(tmp_path / "params.db"), str(tmp_path / "out"),
                            stream=io.BytesIO(slc.tobytes()))
+
+
+def _ci16(slc, peak=3000.0):
+    """Complex samples quantised to interleaved CInt16 (re, im)."""
+    s = peak / np.abs(slc).max()
+    q = np.stack([np.round(slc.real * s), np.round(slc.imag * s)], -1)
+    return np.clip(q, -32768, 32767).astype(np.int16)
+
+
+@pytest.mark.parametrize("odd", [False, True])
+@pytest.mark.parametrize("aligned", [True, False])
+@pytest.mark.parametrize("name", ["small_tops", "small_misreg"])
+def test_deramp_ci16(ctx, name, odd, aligned):
+    """CInt16 bursts deramped straight into complex float: bit for bit the
+    complex-float deramp of the converted samples (8-byte vector path and
+    the per-sample path: odd widths, misaligned input), and against the
+    oracle's deramp (geo2rdr.cu:138-158) of the converted samples."""
+    from sentinel_gpu_processor_amd import _native as N
+    from sentinel_gpu_processor_amd import kernels as K
+    sc = scene(name)
+    b = dict(sc["burst_ref"])
+    raw = _ci16(sc["ref_slc"])
+    if odd:
+        b["samples"] -= 1
+        b["last_valid_sample"] = min(b["last_valid_sample"], b["samples"] - 1)
+        raw = np.ascontiguousarray(raw[:, :-1])
+    L, Nr = b["lines"], b["samples"]
+    print(f"[deramp_ci16] {name} {L}x{Nr} aligned={aligned}", flush=True)
+    g = N.Geometry(ctx, sc["orbit_ref"], b)
+    if aligned:
+        raw_t = _dev(raw)
+    else:
+        buf = torch.zeros(raw.size + 2, dtype=torch.int16, device="cuda")
+        raw_t = buf[2:].view(L, Nr, 2)
+        raw_t.copy_(_dev(raw))
+        assert raw_t.data_ptr() % 8 == 4
+    conv = (raw[..., 0].astype(np.float32) + 1j * raw[..., 1].astype(np.float32)).astype(np.complex64)
+    got = K.deramp_ci16(ctx, g, raw_t)
+    want_dev = K.deramp(ctx, g, _dev(conv))
+    got_np, want_np = _np(got), _np(want_dev)
+    assert np.array_equal(got_np.view(np.uint32), want_np.view(np.uint32))
+    su = G.deramp_setup(sc["orbit_ref"], b)
+    ora = G.deramp(conv, su)
+    assert np.max(np.abs(got_np - ora)) <= 2e-6 * np.abs(ora).max()
+    g.close()
+
+
+def test_deramp_ci16_rejects_bad_buffers(ctx):
+    from sentinel_gpu_processor_amd import _native as N
+    from sentinel_gpu_processor_amd import kernels as K
+    sc = scene("small_tops")
+    b = sc["burst_ref"]
+    L, Nr = b["lines"], b["samples"]
+    g = N.Geometry(ctx, sc["orbit_ref"], b)
+    buf = torch.zeros(L * Nr * 8, dtype=torch.uint8, device="cuda")
+    raw = buf[:L * Nr * 4].view(torch.int16).view(L, Nr, 2)
+    out = buf.view(torch.complex64).view(L, Nr)
+    with pytest.raises(N.S1IfgError, match="overlaps"):
+        K.deramp_ci16(ctx, g, raw, out=out)
+    with pytest.raises(ValueError):
+        K.deramp_ci16(ctx, g, torch.zeros((L, Nr + 1, 2), dtype=torch.int16, device="cuda"))
+    with pytest.raises(TypeError):
+        K.deramp_ci16(ctx, g, torch.zeros((L, Nr, 2), dtype=torch.float32, device="cuda"))
+    g.close()
+
+
+@pytest.mark.parametrize("stored", [True, False])
+def test_geo2rdr_from_tiff_matches_float_stream(tmp_path, stored):
+    """The scene step end to end: two bursts of a CInt16 measurement TIFF
+    inside a zip (stored and deflated) through geo2rdr_from_tiff give the
+    same .gslc bytes as the complex-float stream of the converted samples
+    through geo2rdr_from_db (sentinel_scene.py:164-207 -> geo2rdr.cu
+    --stdin), and a TIFF whose size disagrees with the database is refused."""
+    import zipfile
+
+    from sentinel_gpu_processor_amd import geo2rdr as GR
+    from tests.tiffgen import write_ci16_tiff
+    sc = scene("small_tops")
+    _write_reference_inputs(tmp_path, sc, 0.06)
+    raw = _ci16(sc["ref_slc"])
+    two = np.concatenate([raw, raw])
+    blob = write_ci16_tiff(None, two, rps=1)
+    zp = tmp_path / "S1A_IW_SLC.zip"
+    member = "S1A_IW_SLC.SAFE/measurement/s1a-iw1-slc-vv-001.tiff"
+    with zipfile.ZipFile(zp, "w", zipfile.ZIP_STORED if stored else zipfile.ZIP_DEFLATED) as z:
+        z.writestr(member, blob)
+    db = str(tmp_path / "params.db")
+    got = GR.geo2rdr_from_tiff(db, str(tmp_path / "tif"), str(zp), member)
+    conv = (two[..., 0].astype(np.float32) + 1j * two[..., 1].astype(np.float32)).astype(np.complex64)
+    want = GR.geo2rdr_from_db(db, str(tmp_path / "flt"), stream=io.BytesIO(conv.tobytes()))
+    assert [f.rsplit("_", 1)[1] for f in got] == ["0.gslc", "1.gslc"]
+    assert [f.rsplit("_", 1)[1] for f in want] == ["0.gslc", "1.gslc"]
+    for a, b in zip(got, want):
+        with open(a, "rb") as fa, open(b, "rb") as fb:
+            assert fa.read() == fb.read()
+    bad = tmp_path / "short.tiff"
+    bad.write_bytes(write_ci16_tiff(None, raw, rps=1))
+    with pytest.raises(RuntimeError, match="do not match DB metadata"):
+        GR.geo2rdr_from_tiff(db, str(tmp_path / "bad"), str(bad))
