@@ -417,6 +417,52 @@ int s1ifg_stitch_subswath(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int 
 int s1ifg_merge_subswath(s1ifg_ctx *ctx, const void *src, int nrow, int ncol, int out_float,
                          void *dst, int64_t ld_dst);
 
+/* ---- PS phase similarity (phase_similarity.cu) ----
+ * Stacks are nifg interferograms of nrow x ncol complex floats, image after
+ * image ([nifg][nrow][ncol], DEVICE); candidate masks are int32 [nrow][ncol]
+ * (non-zero = candidate), DEVICE.  Neighbours of a pixel are explored in the
+ * reference's order (s1ifg_ps_scan_offsets) and the first nneigh candidates
+ * inside the image are used; similarities are float sums over the
+ * interferograms in order, divided by nifg.  1 <= nneigh <= 128 (0 is taken
+ * as 1, as the reference's loop does), rdmax <= 2048. */
+
+/* The neighbour offsets (row, column) in exploration order: rings
+ * rdmin+1 .. rdmax-1 of the reference's gap-filled midpoint circle.
+ * Writes min(count, cap) pairs to the HOST array offsets (may be NULL with
+ * cap 0) and returns count (>= 0), or a negative S1IFG_E* code.  Replaces
+ * scan_array (phase_similarity.cu:295-366). */
+int s1ifg_ps_scan_offsets(int rdmin, int rdmax, int32_t *offsets, int cap);
+/* Unit phasors: z / |z|, 0 where |z| <= 1e-8; n complex elements, in may
+ * equal out.  Replaces normalize (phase_similarity.cu:116-134). */
+int s1ifg_ps_normalize(s1ifg_ctx *ctx, const s1ifg_cf32 *in, s1ifg_cf32 *out, int64_t n);
+/* ps[k] = 1 where sim[k] >= threshold, else 0 when overwrite != 0 (else
+ * unchanged); n elements.  Replaces update_ps (phase_similarity.cu:145-157). */
+int s1ifg_ps_update(s1ifg_ctx *ctx, int32_t *ps, const float *sim, float threshold,
+                    int overwrite, int64_t n);
+/* Median similarity of each candidate to its first nneigh candidate
+ * neighbours (element count/2 of the sorted similarities; 0 without
+ * neighbours); 0 for pixels that are not candidates.  stack_norm: an
+ * already normalised stack.  Replaces median_similarity<N>
+ * (phase_similarity.cu:174-233). */
+int s1ifg_ps_median_similarity(s1ifg_ctx *ctx, const s1ifg_cf32 *stack_norm, int nifg,
+                               int nrow, int ncol, const int32_t *ps, int nneigh, int rdmin,
+                               int rdmax, float *sim_median);
+/* 1 for candidates; for the other pixels the largest similarity to their
+ * first nneigh candidate neighbours (0 without any).  Replaces
+ * maximum_similarity (phase_similarity.cu:235-283). */
+int s1ifg_ps_maximum_similarity(s1ifg_ctx *ctx, const s1ifg_cf32 *stack_norm, int nifg,
+                                int nrow, int ncol, const int32_t *ps, int nneigh, int rdmin,
+                                int rdmax, float *sim_max);
+/* The executable's whole sequence on one resident stack: normalise,
+ * median similarity of the candidates, ps <- (median >= med_sim_th)
+ * (updated in place), maximum similarity with the updated candidates.
+ * stack is read only (the normalised copy lives in the workspace).
+ * Replaces similarity() (phase_similarity.cu:375-548) with the whole image
+ * resident (no row batches). */
+int s1ifg_ps_similarity(s1ifg_ctx *ctx, const s1ifg_cf32 *stack, int nifg, int nrow,
+                        int ncol, int32_t *ps, int nneigh, int rdmin, int rdmax,
+                        float med_sim_th, float *med_sim, float *max_sim);
+
 #ifdef __cplusplus
 }
 #endif

@@ -165,3 +165,62 @@ def pair_chain(sc, ref, sec, interp="bilinear", rl=1, cl=4, alpha=0.5, win=5):
     filt = goldstein(ifg, alpha)
     coh = coherence(ifg, p1, p2, win)
     return dict(gr=gr, gs=gs, ifg=ifg, p1=p1, p2=p2, filt=filt, coh=coh)
+
+
+# ------------------------------------------------------- PS phase similarity
+def ps_scan(rdmin, rdmax):
+    lib = load()
+    n = lib.s1c_ps_scan(int(rdmin), int(rdmax), None, 0)
+    out = np.empty((max(n, 0), 2), np.int32)
+    if n > 0:
+        lib.s1c_ps_scan(int(rdmin), int(rdmax), _p(out), n)
+    return out
+
+
+def ps_normalize(stack):
+    z = np.ascontiguousarray(stack, np.complex64)
+    out = np.empty_like(z)
+    load().s1c_ps_normalize(_p(z), _p(out), ctypes.c_long(z.size))
+    return out
+
+
+def _ps_args(zn, ps, offsets):
+    zn = np.ascontiguousarray(zn, np.complex64)
+    ps = np.ascontiguousarray(ps, np.int32)
+    off = np.ascontiguousarray(offsets, np.int32)
+    return zn, ps, off
+
+
+def ps_median(zn, ps, nneigh, offsets):
+    zn, ps, off = _ps_args(zn, ps, offsets)
+    nifg, nrow, ncol = zn.shape
+    out = np.empty((nrow, ncol), np.float32)
+    load().s1c_ps_median(_p(zn), nifg, nrow, ncol, _p(ps), int(nneigh), _p(off), len(off),
+                         _p(out))
+    return out
+
+
+def ps_max(zn, ps, nneigh, offsets):
+    zn, ps, off = _ps_args(zn, ps, offsets)
+    nifg, nrow, ncol = zn.shape
+    out = np.empty((nrow, ncol), np.float32)
+    load().s1c_ps_max(_p(zn), nifg, nrow, ncol, _p(ps), int(nneigh), _p(off), len(off),
+                      _p(out))
+    return out
+
+
+def ps_update(ps, sim, th, overwrite=True):
+    ps = np.array(ps, np.int32, copy=True, order="C")
+    sim = np.ascontiguousarray(sim, np.float32)
+    load().s1c_ps_update(_p(ps), _p(sim), ctypes.c_float(th), int(bool(overwrite)),
+                         ctypes.c_long(ps.size))
+    return ps
+
+
+def ps_similarity(stack, ps, nneigh, rdmin, rdmax, med_sim_th):
+    """(med_sim, max_sim, updated candidates), as oracle.psps.similarity."""
+    off = ps_scan(rdmin, rdmax)
+    zn = ps_normalize(stack)
+    med = ps_median(zn, ps, nneigh, off)
+    ps2 = ps_update(ps, med, med_sim_th, True)
+    return med, ps_max(zn, ps2, nneigh, off), ps2

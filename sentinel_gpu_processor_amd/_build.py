@@ -17,7 +17,7 @@ LIBNAME = "libs1ifg.so"
 ARCH = os.environ.get("S1IFG_ARCH", "gfx950")
 
 SOURCES = ["host.cpp", "k_geom.hip", "k_ifg.hip", "k_goldstein.hip", "k_gold_mfma.hip",
-           "k_batch.hip", "k_adjoint.hip", "k_stitch.hip"]
+           "k_batch.hip", "k_adjoint.hip", "k_stitch.hip", "k_ps.hip"]
 
 HIPFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

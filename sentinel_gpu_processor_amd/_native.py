@@ -13,6 +13,7 @@ import numpy as np
 from . import _build
 
 S1IFG_OK = 0
+S1IFG_EINVAL = -1
 S1IFG_ERANGE = -4
 INTERP_BILINEAR = 0
 INTERP_SINC = 1
@@ -157,6 +158,12 @@ SIGNATURES = {
     "s1ifg_batch_set_launch": [_P, ctypes.POINTER(BatchOpts)],
     "s1ifg_stitch_subswath": [_P, ctypes.POINTER(StitchBurst), _I, _I, _P, _I, _I, _I, _I, _P],
     "s1ifg_merge_subswath": [_P, _P, _I, _I, _I, _P, _I64],
+    "s1ifg_ps_scan_offsets": [_I, _I, _P, _I],
+    "s1ifg_ps_normalize": [_P, _P, _P, _I64],
+    "s1ifg_ps_update": [_P, _P, _P, _F, _I, _I64],
+    "s1ifg_ps_median_similarity": [_P, _P, _I, _I, _I, _P, _I, _I, _I, _P],
+    "s1ifg_ps_maximum_similarity": [_P, _P, _I, _I, _I, _P, _I, _I, _I, _P],
+    "s1ifg_ps_similarity": [_P, _P, _I, _I, _I, _P, _I, _I, _I, _F, _P, _P],
 }
 _VOID = {"s1ifg_ctx_destroy", "s1ifg_geom_destroy", "s1ifg_batch_destroy"}
 _STR = {"s1ifg_last_error"}

@@ -356,6 +356,7 @@ int s1ifg_ctx_create(int device, s1ifg_ctx **out) {
   c->ws_bytes = 0;
   c->ws_limit = -1;
   c->tile_stats = nullptr;
+  c->ps_cache = nullptr;
   const char *lim = std::getenv("S1IFG_WORKSPACE_LIMIT");
   if (lim && *lim) c->ws_limit = std::atoll(lim);
   *out = c;
@@ -370,6 +371,7 @@ void s1ifg_ctx_destroy(s1ifg_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(ctx->ws);
   }
+  ps_cache_free(ctx);
   delete ctx;
 }
 

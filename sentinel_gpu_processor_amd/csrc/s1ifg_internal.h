@@ -101,6 +101,7 @@ struct s1ifg_ctx {
   size_t ws_bytes;
   int64_t ws_limit;    // optional cap (S1IFG_WORKSPACE_LIMIT env), -1 = none
   unsigned long long *tile_stats;  // optional device counters of the pair kernel's tile classes
+  void *ps_cache;      // PS neighbour-offset table on the device (k_ps.hip), or null
 };
 
 struct s1ifg_geom {
@@ -122,4 +123,6 @@ int make_grid(const s1ifg_grid *grid, GridDev *out);
 int make_dem(const s1ifg_dem *dem, DemDev *out);
 int make_offpoly(const double *p, OffPoly *out);
 int device_cu_count(int device);
+// Releases ctx->ps_cache (k_ps.hip).
+void ps_cache_free(s1ifg_ctx *ctx);
 }  // namespace s1ifg

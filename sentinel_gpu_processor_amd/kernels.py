@@ -301,3 +301,91 @@ def to_numpy(t):
 
 def from_numpy(a, device="cuda:0"):
     return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+# ------------------------------------------------------- PS phase similarity
+def ps_scan_offsets(rdmin, rdmax):
+    """Neighbour offsets (dr, dc) in the reference's exploration order
+    (phase_similarity.cu:295-366), an int32 numpy array of shape (n, 2)."""
+    lib = N.load()
+    n = lib.s1ifg_ps_scan_offsets(int(rdmin), int(rdmax), None, 0)
+    N.check(min(n, 0))
+    out = np.empty((n, 2), np.int32)
+    if n:
+        N.check(min(lib.s1ifg_ps_scan_offsets(int(rdmin), int(rdmax),
+                                              out.ctypes.data_as(ctypes.c_void_p), n), 0))
+    return out
+
+
+def _ps_stack(stack):
+    _need(stack, torch.complex64, None, "interferogram stack")
+    if stack.dim() != 3:
+        raise ValueError("stack must be (nifg, nrow, ncol)")
+    return stack.shape
+
+
+def _ps_mask(ps, shape):
+    _need(ps, torch.int32, shape, "candidate mask")
+
+
+def ps_normalize(ctx, stack, out=None):
+    """Unit phasors, 0 where |z| <= 1e-8 (phase_similarity.cu:116-134)."""
+    _need(stack, torch.complex64, None, "stack")
+    if out is None:
+        out = torch.empty_like(stack)
+    _need(out, torch.complex64, stack.shape, "output")
+    N.check(ctx.lib.s1ifg_ps_normalize(ctx.handle, N.ptr(stack), N.ptr(out), stack.numel()))
+    return out
+
+
+def ps_update(ctx, ps, sim, threshold, overwrite=True):
+    """In place: ps = 1 where sim >= threshold, else 0 if overwrite
+    (phase_similarity.cu:145-157)."""
+    _need(ps, torch.int32, None, "candidate mask")
+    _need(sim, torch.float32, ps.shape, "similarity")
+    N.check(ctx.lib.s1ifg_ps_update(ctx.handle, N.ptr(ps), N.ptr(sim), float(threshold),
+                                    int(bool(overwrite)), ps.numel()))
+    return ps
+
+
+def ps_median_similarity(ctx, stack_norm, ps, nneigh, rdmin, rdmax, out=None):
+    nifg, nrow, ncol = _ps_stack(stack_norm)
+    _ps_mask(ps, (nrow, ncol))
+    if out is None:
+        out = torch.empty((nrow, ncol), dtype=torch.float32, device=ps.device)
+    _need(out, torch.float32, (nrow, ncol), "output")
+    N.check(ctx.lib.s1ifg_ps_median_similarity(ctx.handle, N.ptr(stack_norm), nifg, nrow, ncol,
+                                               N.ptr(ps), int(nneigh), int(rdmin), int(rdmax),
+                                               N.ptr(out)))
+    return out
+
+
+def ps_maximum_similarity(ctx, stack_norm, ps, nneigh, rdmin, rdmax, out=None):
+    nifg, nrow, ncol = _ps_stack(stack_norm)
+    _ps_mask(ps, (nrow, ncol))
+    if out is None:
+        out = torch.empty((nrow, ncol), dtype=torch.float32, device=ps.device)
+    _need(out, torch.float32, (nrow, ncol), "output")
+    N.check(ctx.lib.s1ifg_ps_maximum_similarity(ctx.handle, N.ptr(stack_norm), nifg, nrow, ncol,
+                                                N.ptr(ps), int(nneigh), int(rdmin), int(rdmax),
+                                                N.ptr(out)))
+    return out
+
+
+def ps_similarity(ctx, stack, ps, nneigh, rdmin, rdmax, med_sim_th, med=None, mx=None):
+    """The phase_similarity executable's sequence on a resident stack
+    (phase_similarity.cu:375-548): returns (med_sim, max_sim); ps (int32,
+    device) is updated in place to the candidates whose median similarity
+    reaches med_sim_th."""
+    nifg, nrow, ncol = _ps_stack(stack)
+    _ps_mask(ps, (nrow, ncol))
+    if med is None:
+        med = torch.empty((nrow, ncol), dtype=torch.float32, device=ps.device)
+    if mx is None:
+        mx = torch.empty((nrow, ncol), dtype=torch.float32, device=ps.device)
+    _need(med, torch.float32, (nrow, ncol), "med_sim")
+    _need(mx, torch.float32, (nrow, ncol), "max_sim")
+    N.check(ctx.lib.s1ifg_ps_similarity(ctx.handle, N.ptr(stack), nifg, nrow, ncol, N.ptr(ps),
+                                        int(nneigh), int(rdmin), int(rdmax), float(med_sim_th),
+                                        N.ptr(med), N.ptr(mx)))
+    return med, mx

@@ -385,6 +385,36 @@ def aps_field(lines, samples, seed, corr=2000.0, std=1.0, coarse=16):
     return out.astype(np.float32)
 
 
+def ps_stack(nifg, nrow, ncol, seed=0, ps_frac=0.15, extra_frac=0.1, ps_noise=0.35,
+             aps_std=1.5, device=None):
+    """Interferogram stack for PS selection: nifg interferograms of
+    nrow x ncol complex samples, image after image.  A fraction ps_frac of
+    the pixels are stable scatterers, whose phase in interferogram k is a
+    smooth screen (aps_field, aps_std radians) plus Gaussian noise of
+    ps_noise radians and whose amplitude is steady; the others carry a
+    uniform random phase and Rayleigh amplitude.  The candidate mask
+    (int32) marks the scatterers plus a fraction extra_frac of the other
+    pixels.  Returns (stack, candidates, truth): numpy arrays, or torch
+    tensors on `device` (truth, the scatterer mask, stays numpy)."""
+    rng = np.random.default_rng(seed)
+    truth = rng.random((nrow, ncol)) < ps_frac
+    cand = (truth | (rng.random((nrow, ncol)) < extra_frac)).astype(np.int32)
+    stack = np.empty((nifg, nrow, ncol), np.complex64)
+    amp_ps = rng.lognormal(0.0, 0.2, (nrow, ncol)).astype(np.float32)
+    for k in range(nifg):
+        screen = aps_field(nrow, ncol, seed * 1000 + 31 + k, corr=2000.0, std=aps_std)
+        noise = np.where(truth, rng.normal(0.0, ps_noise, (nrow, ncol)),
+                         rng.uniform(-np.pi, np.pi, (nrow, ncol))).astype(np.float32)
+        amp = np.where(truth, amp_ps, rng.rayleigh(0.6, (nrow, ncol))).astype(np.float32)
+        ph = screen + noise
+        stack[k].real = amp * np.cos(ph)
+        stack[k].imag = amp * np.sin(ph)
+    if device is None:
+        return stack, cand, truth
+    import torch
+    return (torch.from_numpy(stack).to(device), torch.from_numpy(cand).to(device), truth)
+
+
 def make_stack(n_dates=20, lines=1500, samples=25000, dem_shape=(2000, 3000),
                dem_kind="fractal", hmax=2500.0, seed=0, gamma0=0.8, bperp_max=150.0,
                aps_corr=2000.0, aps_std=1.0, fidelity="fast", device=None):

@@ -89,3 +89,20 @@ def test_struct_layouts_match_c_compiler(tmp_path):
         cls = structs[cname]
         got = ctypes.sizeof(cls) if field == "sizeof" else getattr(cls, field).offset
         assert got == int(value), (cname, field, got, value)
+
+
+def test_ps_scan_offsets_matches_oracle():
+    """The library's neighbour table (host code, no GPU) against the
+    oracle's restatement of scan_array (phase_similarity.cu:295-366), and
+    its argument checks."""
+    from oracle import psps as P
+    from sentinel_gpu_processor_amd import _native as N
+    from sentinel_gpu_processor_amd import kernels as K
+    import numpy as np
+    for rdmin, rdmax in ((0, 5), (2, 5), (0, 1), (0, 0), (-3, 6), (3, 30), (0, 64), (10, 200)):
+        np.testing.assert_array_equal(K.ps_scan_offsets(rdmin, rdmax),
+                                       P.scan_offsets(max(rdmin, 0), rdmax))
+    lib = N.load()
+    assert lib.s1ifg_ps_scan_offsets(0, 5000, None, 0) == N.S1IFG_ERANGE
+    assert lib.s1ifg_ps_scan_offsets(0, 5, None, 4) == N.S1IFG_EINVAL
+    assert lib.s1ifg_ps_scan_offsets(0, 5, None, 0) == 8 + 12 + 16 + 32
