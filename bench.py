@@ -51,6 +51,8 @@ def parse():
     p.add_argument("--no-stack", action="store_true", help="skip the 20-date stack")
     p.add_argument("--no-stitch", action="store_true", help="skip the IW subswath stitch")
     p.add_argument("--no-ingest", action="store_true", help="skip the CInt16 deramp leg")
+    p.add_argument("--no-ps", action="store_true", help="skip the PS phase-similarity leg")
+    p.add_argument("--ps-steps", type=int, default=10)
     p.add_argument("--stack-dates", type=int, default=20)
     p.add_argument("--gold-mode", default="auto", choices=["auto", "fp32", "fp16_split"])
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -287,6 +289,80 @@ def run_ingest(ctx, geom, ref, dref, reps=20):
     print(f"[bench] ingest: CInt16 deramp {ms16:.3f} ms ({r['ci16_gbs']} GB/s), cf32 deramp "
           f"{ms32:.3f} ms ({r['cf32_gbs']} GB/s); with the pinned H2D copy {pc16:.2f} vs "
           f"{pc32:.2f} ms", file=sys.stderr, flush=True)
+    return r
+
+
+def run_ps(args, nifg=19, nrow=1500, ncol=6250, nneigh=32, rdmin=0, rdmax=16, th=0.5):
+    """PS selection by phase similarity on 19 interferograms of the C2 look
+    grid (1500 x 6250, the stack's 1 x 4 multilooked pairs): one step = the
+    candidate mask restored, then normalise + median similarity + update +
+    maximum similarity (s1ifg_ps_similarity).  Beside it, the C/OpenMP
+    restatement on the same stack (all host threads), whose outputs must be
+    bit-identical."""
+    import numpy as np
+    import torch
+
+    from sentinel_gpu_processor_amd import _native as N
+    from sentinel_gpu_processor_amd import kernels as K
+    from sentinel_gpu_processor_amd import synthetic as S
+
+    t0 = time.perf_counter()
+    z, cand, truth = S.ps_stack(nifg, nrow, ncol, seed=0, ps_frac=0.15, extra_frac=0.1)
+    print(f"[bench] ps: generated {nifg} x {nrow} x {ncol} in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    ctx = N.Context(0)
+    st = torch.from_numpy(z).to(dev)
+    ps0 = torch.from_numpy(cand).to(dev)
+    ps = torch.empty_like(ps0)
+    med = torch.empty((nrow, ncol), dtype=torch.float32, device=dev)
+    mx = torch.empty_like(med)
+
+    def step():
+        ps.copy_(ps0)
+        K.ps_similarity(ctx, st, ps, nneigh, rdmin, rdmax, th, med=med, mx=mx)
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.ps_steps):
+        step()
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / args.ps_steps
+    npix = nrow * ncol
+    alg = npix * (8 * nifg + 4 + 4 + 4 + 4)
+    sel = ps.cpu().numpy() == 1
+    r = dict(workload=f"{nifg} interferograms of {nrow} x {ncol}, candidates {cand.mean():.3f}, "
+                      f"nneigh {nneigh}, rdmin {rdmin}, rdmax {rdmax}, med_sim_th {th}",
+             ms_per_step=round(ms, 3), mpix_s=round(npix / (ms * 1e-3) / 1e6, 1),
+             alg_bytes=alg, achieved_gbs=round(alg / (ms * 1e-3) / 1e9, 1),
+             hbm_frac=round(alg / (ms * 1e-3) / HBM_PEAK, 4),
+             selected=int(sel.sum()), recall=round(float(sel[truth].mean()), 4))
+    print(f"[bench] ps: {ms:.3f} ms per step ({r['mpix_s']} Mpix/s), selected {r['selected']}",
+          file=sys.stderr, flush=True)
+    if not args.no_cpu_baseline:
+        try:
+            from oracle import cport as C
+            t0 = time.perf_counter()
+            cm, cx, cp = C.ps_similarity(z, cand, nneigh, rdmin, rdmax, th)
+            dt = time.perf_counter() - t0
+            same = (np.array_equal(cm.view(np.uint32), med.cpu().numpy().view(np.uint32))
+                    and np.array_equal(cx.view(np.uint32), mx.cpu().numpy().view(np.uint32))
+                    and np.array_equal(cp, ps.cpu().numpy()))
+            r["cpu_baseline"] = dict(value=round(npix / dt / 1e6, 2), unit="Mpix/s",
+                                     cores=C.threads(), kind="port", cpu=cpu_model(),
+                                     seconds=round(dt, 3), bit_identical=bool(same),
+                                     sample="the whole stack through the C/OpenMP restatement "
+                                            "(oracle/c/s1cpu_ps.c)")
+            print(f"[bench] ps cpu ({C.threads()} threads): {dt:.2f} s; GPU outputs "
+                  f"bit-identical: {same}", file=sys.stderr, flush=True)
+        except Exception as e:  # reported, never fatal for the GPU number
+            r["cpu_baseline"] = dict(error=repr(e))
+    ctx.close()
     return r
 
 
@@ -850,6 +926,12 @@ def main():
             stitch = run_stitch(args)
         except Exception as e:  # reported; the one-pair number stands alone
             stitch = dict(error=repr(e))
+    ps = None
+    if not args.no_ps:
+        try:
+            ps = run_ps(args)
+        except Exception as e:  # reported; the one-pair number stands alone
+            ps = dict(error=repr(e))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -882,6 +964,7 @@ def main():
         "stack": stack,
         "stitch": stitch,
         "ingest": res.get("_ingest"),
+        "ps": ps,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -33,6 +33,8 @@ constexpr int kTile = 64;        // pixels per wavefront tile (two mask words)
 constexpr int kPrepChunk = 16;   // interferograms per LDS pass of the preparation
 constexpr int kChunk = 32;       // interferograms held in registers per similarity pass
 constexpr int kMaxRd = 2048;
+constexpr int kXcds = 8;         // workgroups are spread round-robin over the XCDs
+constexpr int kScan = 8;         // neighbour offsets tested per step of the walk
 
 // scan_array (phase_similarity.cu:295-366): rings r = 1 .. rdmax-1 of a
 // midpoint circle walked from (r, 0) towards the diagonal; where the point
@@ -207,34 +209,39 @@ __global__ __launch_bounds__(256) void k_ps_prep(const float2 *__restrict__ stac
 }
 
 // Exclusive scan of the per-tile counts (of set bits, or with invert of the
-// clear ones inside the image) into list offsets; total = list length.
+// clear ones inside the image) into list offsets; total = list length.  One
+// block walks the counts 1024 at a time (coalesced), carrying the running
+// sum: a wavefront prefix by shuffles, then across the 16 wavefronts.
 __global__ __launch_bounds__(1024) void k_ps_scan(const int *__restrict__ counts, int ntiles, int npix,
                                                   int invert, int *__restrict__ base,
                                                   int *__restrict__ total) {
-  __shared__ int sh[1024];
-  const int tid = threadIdx.x;
-  const int per = (ntiles + 1023) / 1024;
-  const int b = min(ntiles, tid * per), e = min(ntiles, b + per);
-  auto cnt = [&](int t) {
-    const int c = counts[t];
-    return invert ? min(kTile, npix - t * kTile) - c : c;
-  };
-  int s = 0;
-  for (int t = b; t < e; ++t) s += cnt(t);
-  sh[tid] = s;
+  __shared__ int wsum[16];
+  __shared__ int carry_s;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid == 0) carry_s = 0;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = tid >= off ? sh[tid - off] : 0;
+  for (int t0 = 0; t0 < ntiles; t0 += 1024) {
+    const int t = t0 + tid;
+    int c = 0;
+    if (t < ntiles) {
+      c = counts[t];
+      if (invert) c = min(kTile, npix - t * kTile) - c;
+    }
+    int x = c;  // inclusive prefix within the wavefront
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
     __syncthreads();
-    sh[tid] += v;
+    int before = carry_s;
+    for (int k = 0; k < w; ++k) before += wsum[k];
+    if (t < ntiles) base[t] = before + x - c;
+    __syncthreads();
+    if (tid == 1023) carry_s = before + x;
     __syncthreads();
   }
-  int run = sh[tid] - s;
-  for (int t = b; t < e; ++t) {
-    base[t] = run;
-    run += cnt(t);
-  }
-  if (tid == 1023) *total = sh[1023];
+  if (tid == 0) *total = carry_s;
 }
 
 // Pixels whose bit is set (clear with invert) into list, in image order.
@@ -310,21 +317,42 @@ __global__ __launch_bounds__(64 * waves_per_block<NN>()) void k_ps_sim(
   const int n = *total;
   const float fn = (float)nifg;
   const int nchunks = MULTI ? (nifg + kChunk - 1) / kChunk : 1;
-  for (long chunk = (long)blockIdx.x * WPB + w; chunk * 64 < n; chunk += (long)gridDim.x * WPB) {
+  // Workgroups go round-robin to the 8 XCDs: XCD x takes the x-th eighth
+  // of the list (a band of rows), so that its L2 serves the neighbourhoods.
+  const long nlist = (n + 63) / 64;
+  const int xcd = blockIdx.x % kXcds;
+  const long band0 = nlist * xcd / kXcds, band1 = nlist * (xcd + 1) / kXcds;
+  const long step = (long)(gridDim.x / kXcds) * WPB;
+  for (long chunk = band0 + (long)(blockIdx.x / kXcds) * WPB + w; chunk < band1; chunk += step) {
     const long e = chunk * 64 + lane;
     const bool live = e < n;
     const int p = list[live ? e : chunk * 64];
     const int r0 = p / ncol, c0 = p - r0 * ncol;
     // phase A: the first nmax candidate neighbours in exploration order
+    // (kScan offsets per step: their mask words are fetched together, then
+    // taken in order)
     int cnt = live ? 0 : nmax;
-    for (int i = 0; i < noff; ++i) {
+    for (int i0 = 0; i0 < noff; i0 += kScan) {
       if (__ballot(cnt < nmax) == 0) break;
-      const int o = tab[i];
-      const int r = r0 + (o >> 16), c = c0 + (int)(short)(o & 0xffff);
-      if (cnt < nmax && (unsigned)r < (unsigned)nrow && (unsigned)c < (unsigned)ncol) {
-        const int q = r * ncol + c;
-        if ((bits[q >> 5] >> (q & 31)) & 1u) {
-          Lq[cnt * 64 + lane] = q;
+      int qv[kScan];
+      unsigned wv[kScan];
+#pragma unroll
+      for (int u = 0; u < kScan; ++u) {
+        qv[u] = -1;
+        wv[u] = 0;
+        if (i0 + u < noff) {
+          const int o = tab[i0 + u];
+          const int r = r0 + (o >> 16), c = c0 + (int)(short)(o & 0xffff);
+          if ((unsigned)r < (unsigned)nrow && (unsigned)c < (unsigned)ncol) {
+            qv[u] = r * ncol + c;
+            wv[u] = bits[qv[u] >> 5];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kScan; ++u) {
+        if (qv[u] >= 0 && cnt < nmax && ((wv[u] >> (qv[u] & 31)) & 1u)) {
+          Lq[cnt * 64 + lane] = qv[u];
           ++cnt;
         }
       }
@@ -479,7 +507,9 @@ int launch_sim_nn(s1ifg_ctx *ctx, const PsWork &wk, int nifg, int nrow, int ncol
   const long chunks = (wk.npix + 63) / 64;
   const long want = (chunks + WPB - 1) / WPB;
   const long cap = (long)device_cu_count(ctx->device) * 16;
-  const unsigned grid = (unsigned)std::max(1L, std::min(want, cap));
+  // a multiple of the XCD count (each XCD works on its own band of the list)
+  const long g = std::max(1L, std::min(want, cap));
+  const unsigned grid = (unsigned)((g + kXcds - 1) / kXcds * kXcds);
   auto kern = multi ? k_ps_sim<NN, MEDIAN, true> : k_ps_sim<NN, MEDIAN, false>;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * WPB), lds, ctx->stream,
                      reinterpret_cast<const float4 *>(wk.zt), nifg, wk.stride / 2, nrow, ncol,
