@@ -16,11 +16,19 @@ Contract kept from the reference:
 - all-or-nothing tuning arguments (crossmul_daemon.cu:1762-1785).
 
 Design: an I/O thread pool reads the needed rows of both files ahead of the
-GPU (bounded by the slot count); the GPU stage uploads the row blocks and
-runs the fused crossmul + looks kernel directly on the file layout (the
-column crop is a pointer offset with the file's row stride, so no host
-crop stage exists); a writer thread stores results.  A two-entry image
-cache serves consecutive tasks that share an image (the task order from
+GPU (bounded by the slot count).  The GPU stage has ``streams`` lanes
+(the reference's per-GPU streams and task slots, crossmul_daemon.cu:127-261,
+921-975): each lane owns a HIP stream (default priority), a library
+context bound to it, page-locked host staging buffers and device buffers,
+all grown on demand and reused.  A task on a lane is: copy the row blocks
+into the lane's page-locked buffers, asynchronous host-to-device copies,
+the fused crossmul + looks kernel directly on the file layout (the column
+crop is a pointer offset with the file's row stride), the optional phase
+kernel, an asynchronous device-to-host copy of the looks, and an event.
+The loop polls the lanes' events (as the reference polls its streams) and
+hands finished looks to a writer thread, so one task's transfers overlap
+another's kernel.  A two-entry image cache serves consecutive tasks that
+share an image (the task order from
 ``interfere._reorder_tasks_for_cache_locality`` maximises that).
 """
 
@@ -107,12 +115,13 @@ def _read_rows(path, h, r0, r1):
 
 class CrossmulDriver:
     def __init__(self, rowlook=1, collook=1, out_float=False, io_workers=2, max_slots=4,
-                 verbose=False, out=None, err=None, progress_every=2.0):
+                 verbose=False, out=None, err=None, progress_every=2.0, streams=2):
         self.rl = int(rowlook)
         self.cl = int(collook)
         self.out_float = bool(out_float)
         self.io_workers = max(1, int(io_workers))
         self.max_slots = max(1, int(max_slots))
+        self.streams = max(1, int(streams))
         self.verbose = verbose
         self.out = out or sys.stdout
         self.err = err or sys.stderr
@@ -150,80 +159,165 @@ class CrossmulDriver:
         return dict(out=out, a=a, b=b, la=left - int(h1[2]), lb=left - int(h2[2]),
                     nrow=bottom - top, ncol=right - left, header=header)
 
-    def _gpu(self, ctx, p):
-        import torch
-        from . import _native as N
-        from . import kernels as K
-        dev = torch.device(f"cuda:{ctx.device}")
-        ta = torch.from_numpy(np.ascontiguousarray(p["a"])).to(dev)
-        tb = torch.from_numpy(np.ascontiguousarray(p["b"])).to(dev)
-        nr, nc = p["nrow"] // self.rl, p["ncol"] // self.cl
-        ifg = torch.empty((nr, nc), dtype=torch.complex64, device=dev)
-        pa = ctypes.c_void_p(ta.data_ptr() + 8 * p["la"])
-        pb = ctypes.c_void_p(tb.data_ptr() + 8 * p["lb"])
-        N.check(ctx.lib.s1ifg_crossmul_looks(ctx.handle, pa, ta.shape[1], pb, tb.shape[1],
-                                             p["nrow"], p["ncol"], self.rl, self.cl,
-                                             N.ptr(ifg), None, None))
-        res = K.phase(ctx, ifg) if self.out_float else ifg
-        return res.cpu().numpy()
-
     def run(self, tasks, ctx=None):
+        """Process every task; returns (succeeded, failed).  ctx: the device
+        is taken from it (default: the runtime's); the lanes have their own
+        contexts and streams."""
+        import torch
         from .runtime import context
-        ctx = ctx or context()
+        base = ctx or context()
+        lanes = [_Lane(base.device) for _ in range(self.streams)]
         n = len(tasks)
-        ok = failed = 0
+        self.ok = self.failed = 0
         t0 = time.time()
         last = t0
-        writes = []
+        self._writes = collections.deque()
         with cf.ThreadPoolExecutor(self.io_workers) as io, cf.ThreadPoolExecutor(1) as wr:
+            self._wr = wr
             pending = collections.deque()
             it = iter(tasks)
-            for _ in range(self.max_slots):
+            for _ in range(max(self.max_slots, self.streams)):
                 t = next(it, None)
                 if t is None:
                     break
                 pending.append((t, io.submit(self._prepare, t)))
-            while pending:
-                task, fut = pending.popleft()
-                t = next(it, None)
-                if t is not None:
-                    pending.append((t, io.submit(self._prepare, t)))
-                try:
-                    p = fut.result()
-                    data = self._gpu(ctx, p)
-                    h = p["header"]
-                    writes.append((task, wr.submit(save_cropped, p["out"], data, h[0], h[1], h[2],
-                                                   h[3], h[4], h[5])))
-                except TaskFail as e:
-                    failed += 1
-                    self._emit(f"[crossmul_daemon] FAIL {task[2]} - {e}", err=True)
-                except Exception as e:  # GPU / launch errors fail the task, not the run
-                    failed += 1
-                    self._emit(f"[crossmul_daemon] FAIL {task[2]} - GPU error: {e}", err=True)
-                # collect finished writes in order
-                while writes and writes[0][1].done():
-                    wt, wf = writes.pop(0)
+            while pending or any(l.busy for l in lanes):
+                progressed = False
+                for lane in lanes:  # retire finished lanes (cudaStreamQuery-style)
+                    if lane.busy and lane.done():
+                        self._retire(lane)
+                        progressed = True
+                for lane in lanes:  # start ready tasks on free lanes, in task order
+                    if lane.busy or not pending or not pending[0][1].done():
+                        continue
+                    task, fut = pending.popleft()
+                    t = next(it, None)
+                    if t is not None:
+                        pending.append((t, io.submit(self._prepare, t)))
+                    progressed = True
                     try:
-                        wf.result()
-                        ok += 1
-                        self._emit(f"OK {wt[2]}")
-                    except Exception as e:
-                        failed += 1
-                        self._emit(f"[crossmul_daemon] FAIL {wt[2]} - write error: {e}", err=True)
+                        lane.launch(task, fut.result(), self.rl, self.cl, self.out_float)
+                    except TaskFail as e:
+                        self._fail(task, str(e))
+                    except Exception as e:  # GPU / launch errors fail the task, not the run
+                        lane.reset()
+                        self._fail(task, f"GPU error: {e}")
+                self._collect(block=False)
+                if not progressed:
+                    busy = [l for l in lanes if l.busy]
+                    if busy:
+                        busy[0].event.synchronize()
+                    elif pending:
+                        cf.wait([pending[0][1]], return_when=cf.FIRST_COMPLETED)
                 now = time.time()
                 if now - last >= self.progress_every:
                     last = now
-                    self._emit(f"PROGRESS {ok + failed} {failed} {n} {int(now - t0)}")
-            for wt, wf in writes:
-                try:
-                    wf.result()
-                    ok += 1
-                    self._emit(f"OK {wt[2]}")
-                except Exception as e:
-                    failed += 1
-                    self._emit(f"[crossmul_daemon] FAIL {wt[2]} - write error: {e}", err=True)
-        self._emit(f"SUMMARY {n - failed} {failed} {n} {int(time.time() - t0)}")
-        return n - failed, failed
+                    self._emit(f"PROGRESS {self.ok + self.failed} {self.failed} {n} {int(now - t0)}")
+            self._collect(block=True)
+        for lane in lanes:
+            lane.close()
+        self._emit(f"SUMMARY {n - self.failed} {self.failed} {n} {int(time.time() - t0)}")
+        return n - self.failed, self.failed
+
+    def _fail(self, task, why):
+        self.failed += 1
+        self._emit(f"[crossmul_daemon] FAIL {task[2]} - {why}", err=True)
+
+    def _retire(self, lane):
+        task, p, data = lane.take()
+        h = p["header"]
+        self._writes.append((task, self._wr.submit(save_cropped, p["out"], data, h[0], h[1], h[2],
+                                                   h[3], h[4], h[5])))
+
+    def _collect(self, block):
+        while self._writes and (block or self._writes[0][1].done()):
+            wt, wf = self._writes.popleft()
+            try:
+                wf.result()
+                self.ok += 1
+                self._emit(f"OK {wt[2]}")
+            except Exception as e:
+                self._fail(wt, f"write error: {e}")
+
+
+class _Lane:
+    """One GPU lane: a stream, a context bound to it, page-locked staging
+    and device buffers (grown on demand), and the task in flight."""
+
+    def __init__(self, device):
+        import torch
+        from . import _native as N
+        self.dev = torch.device(f"cuda:{device}")
+        self.stream = torch.cuda.Stream(device=self.dev)
+        self.ctx = N.Context(device, stream=self.stream)
+        self.host = {}
+        self.devbuf = {}
+        self.busy = False
+        self.event = None
+
+    def _buf(self, store, key, n, dtype, pinned):
+        import torch
+        b = store.get(key)
+        if b is None or b.numel() < n or b.dtype != dtype:
+            if pinned:
+                b = torch.empty(max(n, 1), dtype=dtype, pin_memory=True)
+            else:
+                with torch.cuda.stream(self.stream):
+                    b = torch.empty(max(n, 1), dtype=dtype, device=self.dev)
+            store[key] = b
+        return b[:n]
+
+    def launch(self, task, p, rl, cl, out_float):
+        import torch
+        from . import _native as N
+        from . import kernels as K
+        a, b = p["a"], p["b"]
+        nr, nc = p["nrow"] // rl, p["ncol"] // cl
+        ha = self._buf(self.host, "a", a.size, torch.complex64, True)
+        hb = self._buf(self.host, "b", b.size, torch.complex64, True)
+        np.copyto(ha.numpy().reshape(a.shape), a)
+        np.copyto(hb.numpy().reshape(b.shape), b)
+        da = self._buf(self.devbuf, "a", a.size, torch.complex64, False)
+        db = self._buf(self.devbuf, "b", b.size, torch.complex64, False)
+        ifg = self._buf(self.devbuf, "ifg", nr * nc, torch.complex64, False).view(nr, nc)
+        odt = torch.float32 if out_float else torch.complex64
+        ho = self._buf(self.host, "out", nr * nc, odt, True)
+        with torch.cuda.stream(self.stream):
+            da.copy_(ha, non_blocking=True)
+            db.copy_(hb, non_blocking=True)
+            pa = ctypes.c_void_p(da.data_ptr() + 8 * p["la"])
+            pb = ctypes.c_void_p(db.data_ptr() + 8 * p["lb"])
+            N.check(self.ctx.lib.s1ifg_crossmul_looks(
+                self.ctx.handle, pa, a.shape[1], pb, b.shape[1], p["nrow"], p["ncol"], rl, cl,
+                N.ptr(ifg), None, None))
+            if out_float:
+                ph = self._buf(self.devbuf, "ph", nr * nc, torch.float32, False).view(nr, nc)
+                res = K.phase(self.ctx, ifg, out=ph)
+            else:
+                res = ifg
+            ho.copy_(res.reshape(-1), non_blocking=True)
+            self.event = torch.cuda.Event()
+            self.event.record(self.stream)
+        self.task, self.p, self.shape, self.out = task, p, (nr, nc), ho
+        self.busy = True
+
+    def done(self):
+        return self.event.query()
+
+    def take(self):
+        self.event.synchronize()
+        data = self.out.numpy().reshape(self.shape).copy()
+        self.busy = False
+        return self.task, self.p, data
+
+    def reset(self):
+        if self.event is not None:
+            self.event.synchronize()
+        self.busy = False
+
+    def close(self):
+        self.stream.synchronize()
+        self.ctx.close()
 
 
 def main(argv=None):
@@ -255,7 +349,9 @@ def main(argv=None):
     tasks = read_tasks(a.tasks_file)
     io_workers = a.io_workers if a.io_workers > 0 else min(4, os.cpu_count() or 1)
     slots = a.max_slots if a.max_slots > 0 else 4
-    drv = CrossmulDriver(a.rowlook, a.collook, a.out_float, io_workers, slots, a.verbose)
+    streams = a.streams_per_gpu if a.streams_per_gpu > 0 else 2
+    drv = CrossmulDriver(a.rowlook, a.collook, a.out_float, io_workers, slots, a.verbose,
+                         streams=streams)
     _, failed = drv.run(tasks)
     return 1 if failed else 0
 

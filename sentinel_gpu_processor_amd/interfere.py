@@ -174,9 +174,10 @@ def _run_crossmul_daemon(task_items, rowlook, collook, out_float, io_workers, cp
     tasks = _reorder_tasks_for_cache_locality(task_items)
     io = io_workers if io_workers and io_workers > 0 else min(4, os.cpu_count() or 1)
     slots = max_slots if max_slots and max_slots > 0 else 4
+    streams = streams_per_gpu if streams_per_gpu and streams_per_gpu > 0 else 2
     with open(os.devnull, "w") as devnull:
         drv = CrossmulDriver(rowlook, collook, out_float, io, slots, verbose,
-                             out=None if verbose else devnull)
+                             out=None if verbose else devnull, streams=streams)
         return drv.run(tasks)
 
 

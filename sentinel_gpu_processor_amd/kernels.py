@@ -120,10 +120,12 @@ def looks(ctx, img, rl, cl):
     return out
 
 
-def phase(ctx, z):
+def phase(ctx, z, out=None):
     """atan2 of a complex image (sario.cu:231-239)."""
     _need(z, torch.complex64, None, "image")
-    out = _new(tuple(z.shape), torch.float32, z.device)
+    if out is None:
+        out = _new(tuple(z.shape), torch.float32, z.device)
+    _need(out, torch.float32, tuple(z.shape), "phase")
     N.check(ctx.lib.s1ifg_phase(ctx.handle, N.ptr(z), z.numel(), N.ptr(out)))
     return out
 

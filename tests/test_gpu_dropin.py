@@ -192,3 +192,36 @@ def test_sario_looks_on_gpu():
     assert np.allclose(SA.cpxlooks(a, 3, 4), OI.cpx_looks(a, 3, 4), rtol=1e-5, atol=1e-6)
     assert np.allclose(SA.powlooks(a, 3, 4), OI.powlooks(a, 3, 4), rtol=1e-5, atol=1e-6)
     assert np.allclose(SA.powlooks(a, 1, 1), OI.powlooks(a, 1, 1), rtol=1e-5, atol=1e-6)
+
+
+def test_daemon_lanes_match_single_lane(gslc, tmp_path):
+    """The driver with 1 and with 3 stream lanes (overlapped transfers and
+    kernels, crossmul_daemon.cu:921-975): the same bytes in every output,
+    complex and phase, with a failing task in the middle and windows of
+    different sizes so that the lanes' buffers grow."""
+    import io
+
+    from sentinel_gpu_processor_amd.daemon import CrossmulDriver
+    from sentinel_gpu_processor_amd.sario import CroppedImage, save_cropped
+    sc, rsc, d, names = gslc
+    a = CroppedImage.from_file(names["ref"], load_data=True)
+    tasks = []
+    for k in range(7):
+        cut = str(tmp_path / f"cut{k}.gslc")
+        dr = 3 * k
+        save_cropped(cut, a.data[dr:], a.nrow0, a.ncol0, a.left, a.top + dr, a.right, a.bottom)
+        tasks.append((names["ref"], cut, f"OUT{k}.int"))
+    tasks.insert(3, (names["ref"], "/nonexistent.gslc", "OUTbad.int"))
+    for out_float in (False, True):
+        res = {}
+        for streams in (1, 3):
+            od = tmp_path / f"s{streams}_{int(out_float)}"
+            od.mkdir()
+            tk = [(r, s_, str(od / o)) for r, s_, o in tasks]
+            so, se = io.StringIO(), io.StringIO()
+            drv = CrossmulDriver(1, 4, out_float, 2, 4, out=so, err=se, streams=streams)
+            assert drv.run(tk) == (7, 1)
+            assert "SUMMARY 7 1 8" in so.getvalue() and "FAIL" in se.getvalue()
+            res[streams] = {o: open(str(od / o), "rb").read() for _, _, o in tasks
+                            if o != "OUTbad.int"}
+        assert res[1] == res[3]
