@@ -87,6 +87,20 @@ def test_stitch_carry_empty_and_ragged(out_float):
     assert woffs[2] == woffs[1] and woffs[3] == woffs[2]
 
 
+@pytest.mark.parametrize("out_float", [False, True])
+def test_stitch_even_offsets(out_float):
+    """Even mosaic width with even, non-zero burst column offsets: the
+    16-byte complex kernels (placement of burst 0 at column 2 of the box,
+    later bursts at columns 4 and 6), and the same layout in phase images."""
+    rng = np.random.default_rng(14)
+    spans = [(0, 70), (60, 140), (128, 200), (190, 260)]
+    cols = [(2, 334), (4, 330), (6, 334), (0, 334)]
+    offs = [0.3, -1.1, 2.4, -0.6]
+    ids = [("e", str(k)) for k in range(4)]
+    bursts, lst = _make(rng, 334, spans, offs, ids, out_float, cols=cols)
+    _check(bursts, lst, ids, out_float)
+
+
 @pytest.mark.parametrize("n_ov", [1, 2, 3, 64, 65])
 def test_stitch_small_overlap_counts(n_ov):
     """Overlaps of 1, 2, 3, 64 and 65 pixels: single, even and odd medians."""
