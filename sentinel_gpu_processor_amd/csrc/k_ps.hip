@@ -21,6 +21,8 @@
 // (separately rounded products and sums), so results equal the float CPU
 // restatement of the same loops.
 #include <algorithm>
+#include <climits>
+#include <cstdlib>
 #include <vector>
 
 #include "s1ifg_internal.h"
@@ -34,7 +36,7 @@ constexpr int kPrepChunk = 16;   // interferograms per LDS pass of the preparati
 constexpr int kChunk = 32;       // interferograms held in registers per similarity pass
 constexpr int kMaxRd = 2048;
 constexpr int kXcds = 8;         // workgroups are spread round-robin over the XCDs
-constexpr int kScan = 8;         // neighbour offsets tested per step of the walk
+constexpr int kScan = 8;         // neighbour offsets tested per step of the walk (ov[] below)
 
 // scan_array (phase_similarity.cu:295-366): rings r = 1 .. rdmax-1 of a
 // midpoint circle walked from (r, 0) towards the diagonal; where the point
@@ -95,7 +97,7 @@ void scan_offsets(int rdmin, int rdmax, std::vector<int32_t> &out) {
 }
 
 struct PsCache {
-  int rdmin = 0, rdmax = -1, noff = 0;
+  int rdmin = 0, rdmax = -1, noff = 0, extent = 0;
   int32_t *dev = nullptr;
   size_t cap = 0;
   std::vector<int32_t> host;  // source of the last upload, kept until the next change
@@ -104,7 +106,8 @@ struct PsCache {
 // The offset table for (rdmin, rdmax) on the device, built and uploaded
 // when it changes (the stream is drained first: queued kernels may still
 // read the previous table).
-int ps_table(s1ifg_ctx *ctx, int rdmin, int rdmax, const int32_t **tab, int *noff) {
+int ps_table(s1ifg_ctx *ctx, int rdmin, int rdmax, const int32_t **tab, int *noff,
+             int *extent) {
   rdmin = std::max(rdmin, 0);
   PsCache *c = static_cast<PsCache *>(ctx->ps_cache);
   if (!c) {
@@ -114,7 +117,11 @@ int ps_table(s1ifg_ctx *ctx, int rdmin, int rdmax, const int32_t **tab, int *nof
   if (c->rdmax != rdmax || c->rdmin != rdmin) {
     S1_HIP(hipStreamSynchronize(ctx->stream));
     scan_offsets(rdmin, rdmax, c->host);
-    const size_t n = c->host.size();
+    const size_t n0 = c->host.size();
+    // padded with zero entries to a whole number of walk steps (kScan), so
+    // that a step reads its entries in one go
+    const size_t n = (n0 + kScan - 1) / kScan * kScan;
+    c->host.resize(n, 0);
     if (n > c->cap) {
       if (c->dev) S1_HIP(hipFree(c->dev));
       c->dev = nullptr;
@@ -127,10 +134,14 @@ int ps_table(s1ifg_ctx *ctx, int rdmin, int rdmax, const int32_t **tab, int *nof
                             ctx->stream));
     c->rdmin = rdmin;
     c->rdmax = rdmax;
-    c->noff = (int)n;
+    c->noff = (int)n0;
+    c->extent = 0;
+    for (int32_t o : c->host)
+      c->extent = std::max(c->extent, std::max(std::abs(o >> 16), std::abs((int)(int16_t)(o & 0xffff))));
   }
   *tab = c->dev;
   *noff = c->noff;
+  *extent = c->extent;
   return S1IFG_OK;
 }
 
@@ -427,6 +438,325 @@ __global__ __launch_bounds__(64 * waves_per_block<NN>()) void k_ps_sim(
   }
 }
 
+// LDS-tiled similarities (the form used when the series fit): a block owns
+// a kPT x kPT tile of pixels.  It stages into LDS the series of the
+// eligible neighbours (candidates / kept PS) inside the tile grown by a
+// halo of up to kHalo pixels, in window order, up to `cap` of them (the
+// rest are read from global memory), with a map window pixel -> slot.
+// Phase A: each of the tile's centres (candidates for the median pass,
+// non-PS pixels for the maximum pass) walks the offsets in the reference's
+// order and records its first nmax eligible neighbours (an LDS slot, or a
+// global pixel for those outside the staged window).  Phase B: all 256
+// threads compute the (centre, neighbour) similarities, consecutive
+// threads on one centre's neighbours, and store them over the references.
+// Phase C: each centre reduces its similarities (median by the
+// compare-exchange chain, or maximum).  Every similarity is the same float
+// sequence as in k_ps_sim, whatever memory the series come from.
+constexpr int kPT = 16;      // tile side (pixels)
+constexpr int kHalo = 10;    // staged halo around the tile (pixels)
+constexpr int kTileLds = 76 * 1024;  // LDS per block (two blocks per CU, with margin)
+constexpr int kStageLoads = 8;       // staging loads in flight per thread
+
+// Per-centre rows of NN + 1 words (odd stride: a wavefront's lanes on
+// consecutive centres, or on consecutive neighbours of one centre, fall on
+// different banks).
+template <int NN>
+__host__ __device__ constexpr int tile_fixed_lds(int ww) {
+  return ((256 * (NN + 1) * 4 + 15) & ~15) + ((ww * ww * 2 + 15) & ~15);
+}
+
+template <bool CL, bool NL>
+__device__ __forceinline__ float ps_dot(const float4 *__restrict__ a, const float4 *__restrict__ b,
+                                        int nifg, int s4, float fn) {
+#pragma clang fp contract(off)
+  float acc = 0.f;
+  for (int jj = 0; jj < s4; ++jj) {
+    const float4 u = a[jj], v = b[jj];
+    acc += u.x * v.x + u.y * v.y;
+    if (2 * jj + 1 < nifg) acc += u.z * v.z + u.w * v.w;
+  }
+  return acc / fn;
+}
+
+template <int NN, bool MEDIAN>
+__global__ __launch_bounds__(256) void k_ps_tile(const float4 *__restrict__ zt, int nifg, int s4,
+                                                 int nrow, int ncol, const unsigned *__restrict__ bits,
+                                                 const int *__restrict__ tab, int noff, int nmax,
+                                                 int halo, int cap, int per_xcd,
+                                                 float *__restrict__ out) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ int cpix[256], ccnt[256], wsum[4], spix_dummy;
+  const int ww = kPT + 2 * halo;
+  constexpr int PS = NN + 1;
+  int *pr = reinterpret_cast<int *>(lds_raw);  // [256][PS]
+  short *smap = reinterpret_cast<short *>(lds_raw + ((256 * PS * 4 + 15) & ~15));  // [ww][ww]
+  const int ss = s4 | 1;  // staged series stride (odd: slots spread over the banks)
+  float4 *ser = reinterpret_cast<float4 *>(lds_raw + tile_fixed_lds<NN>(ww));  // [cap][ss]
+  int *spix = reinterpret_cast<int *>(ser + (size_t)cap * ss);      // [cap]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tiles_c = (ncol + kPT - 1) / kPT, tiles_r = (nrow + kPT - 1) / kPT;
+  // XCD-contiguous tiles: workgroup b runs on XCD b % 8
+  const int t = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (t >= tiles_r * tiles_c) return;  // whole block, before any barrier
+  const int tr = t / tiles_c, tc = t - tr * tiles_c;
+  const int r0 = tr * kPT - halo, c0 = tc * kPT - halo;
+  (void)spix_dummy;
+  // 1. window slot map, the staged pixels in window order
+  int base = 0;
+  for (int i0 = 0; i0 < ww * ww; i0 += 256) {
+    const int i = i0 + tid;
+    int q = -1;
+    bool el = false;
+    if (i < ww * ww) {
+      const int r = r0 + i / ww, c = c0 + i % ww;
+      if ((unsigned)r < (unsigned)nrow && (unsigned)c < (unsigned)ncol) {
+        q = r * ncol + c;
+        el = (bits[q >> 5] >> (q & 31)) & 1u;
+      }
+    }
+    const unsigned long long m = __ballot(el);
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int before = base;
+    for (int k = 0; k < w; ++k) before += wsum[k];
+    const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    const int slot = before + __popcll(m & ((1ull << lane) - 1));
+    if (i < ww * ww) smap[i] = !el ? (short)-1 : (slot < cap ? (short)slot : (short)-2);
+    if (el && slot < cap) spix[slot] = q;
+    base += total;
+    __syncthreads();
+  }
+  const int E = min(base, cap);
+  for (int u0 = tid; u0 < E * s4; u0 += kStageLoads * 256) {
+    float4 v[kStageLoads];  // loads in flight before their stores
+#pragma unroll
+    for (int k = 0; k < kStageLoads; ++k) {
+      const int u = u0 + k * 256;
+      if (u < E * s4) {
+        const int sl = u / s4;
+        v[k] = zt[(size_t)spix[sl] * s4 + (u - sl * s4)];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kStageLoads; ++k) {
+      const int u = u0 + k * 256;
+      if (u < E * s4) {
+        const int sl = u / s4;
+        ser[sl * ss + (u - sl * s4)] = v[k];
+      }
+    }
+  }
+  // 2. the tile's centres, compacted
+  {
+    const int ty = tid / kPT, tx = tid - ty * kPT;
+    const int r = tr * kPT + ty, c = tc * kPT + tx;
+    bool isc = false;
+    int q = -1;
+    if (r < nrow && c < ncol) {
+      q = r * ncol + c;
+      const bool el = (bits[q >> 5] >> (q & 31)) & 1u;
+      isc = MEDIAN ? el : !el;
+    }
+    const unsigned long long m = __ballot(isc);
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int before = 0;
+    for (int k = 0; k < w; ++k) before += wsum[k];
+    if (isc) cpix[before + __popcll(m & ((1ull << lane) - 1))] = q;
+  }
+  __syncthreads();  // cpix written
+  const int nc = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  // 3. phase A: first nmax eligible neighbours in exploration order
+  if (tid < nc) {
+    const int q = cpix[tid];
+    const int r = q / ncol, c = q - r * ncol;
+    const int wr = r - r0, wc = c - c0;
+    int cnt = 0;
+    int *mine = pr + tid * PS;
+    const int cw = wr * ww + wc;  // the centre's window index
+    // kScan offsets per step, branch-free so that their table entries,
+    // slots and mask words are all in flight together, then taken in
+    // order.  An offset within the halo (a uniform test) stays inside the
+    // window for every centre of the tile (the map holds -1 for window
+    // pixels outside the image); beyond it the mask word decides.  Reads
+    // that a step does not need use harmless addresses (the centre).
+    for (int i0 = 0; i0 < noff && cnt < nmax; i0 += kScan) {
+      int ref[kScan], qv[kScan], sv[kScan];
+      unsigned wv[kScan];
+      bool inh[kScan], inimg[kScan], ok[kScan];
+      int ov[kScan];  // the step's table entries (the table is padded to whole steps)
+      {
+        const int4 *t4 = reinterpret_cast<const int4 *>(tab + i0);
+        const int4 a = t4[0], b = t4[1];
+        ov[0] = a.x; ov[1] = a.y; ov[2] = a.z; ov[3] = a.w;
+        ov[4] = b.x; ov[5] = b.y; ov[6] = b.z; ov[7] = b.w;
+      }
+      bool allh = true;  // the whole step inside the halo (uniform)
+#pragma unroll
+      for (int u = 0; u < kScan; ++u) {
+        const int dr = ov[u] >> 16, dc = (int)(short)(ov[u] & 0xffff);
+        allh = allh && abs(dr) <= halo && abs(dc) <= halo;
+      }
+      if (allh) {
+        // the window map alone (the usual case: the rings within the halo
+        // come first)
+#pragma unroll
+        for (int u = 0; u < kScan; ++u) {
+          const int dr = ov[u] >> 16, dc = (int)(short)(ov[u] & 0xffff);
+          sv[u] = smap[cw + dr * ww + dc];
+        }
+#pragma unroll
+        for (int u = 0; u < kScan; ++u) {
+          const int dr = ov[u] >> 16, dc = (int)(short)(ov[u] & 0xffff);
+          const int gref = -(q + dr * ncol + dc) - 1;
+          ref[u] = (i0 + u >= noff) ? INT_MIN
+                                    : (sv[u] >= 0 ? sv[u] : (sv[u] == -2 ? gref : INT_MIN));
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < kScan; ++u) {
+          ok[u] = i0 + u < noff;
+          const int o = ov[u];
+          const int dr = o >> 16, dc = (int)(short)(o & 0xffff);
+          inh[u] = abs(dr) <= halo && abs(dc) <= halo;
+          const int rr = r + dr, cc = c + dc;
+          inimg[u] = (unsigned)rr < (unsigned)nrow && (unsigned)cc < (unsigned)ncol;
+          qv[u] = inimg[u] ? rr * ncol + cc : q;
+          sv[u] = smap[cw + (inh[u] ? dr * ww + dc : 0)];
+          wv[u] = bits[qv[u] >> 5];
+        }
+#pragma unroll
+        for (int u = 0; u < kScan; ++u) {
+          const bool elg = inimg[u] && ((wv[u] >> (qv[u] & 31)) & 1u);
+          const int gref = -qv[u] - 1;
+          const int lref = sv[u] >= 0 ? sv[u] : (sv[u] == -2 ? gref : INT_MIN);
+          ref[u] = !ok[u] ? INT_MIN : (inh[u] ? lref : (elg ? gref : INT_MIN));
+        }
+      }
+      // record the step's eligible neighbours in order while fewer than
+      // nmax: positions by a running count, the rest written to the row's
+      // spare entry NN (never read)
+      int pos = cnt;
+#pragma unroll
+      for (int u = 0; u < kScan; ++u) {
+        const bool take = ref[u] != INT_MIN && pos < nmax;
+        mine[take ? pos : NN] = ref[u];
+        pos += take ? 1 : 0;
+      }
+      cnt = pos;
+    }
+    ccnt[tid] = cnt;
+  }
+  __syncthreads();
+  const float fn = (float)nifg;
+  if (!MEDIAN && s4 <= kChunk / 2) {
+    // maximum pass, series of up to kChunk interferograms: one thread per
+    // centre (non-PS pixels, not staged), its series in registers, read
+    // once; the neighbours from LDS (or global memory past the window).
+    // (The median pass has few centres per tile; it spreads the
+    // (centre, neighbour) pairs over all threads below instead.)
+    if (tid < nc) {
+      const int q = cpix[tid];
+      float4 cen[kChunk / 2];
+#pragma unroll
+      for (int jj = 0; jj < kChunk / 2; ++jj)
+        cen[jj] = jj < s4 ? zt[(size_t)q * s4 + jj] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const int cnt = ccnt[tid];
+      float best = -1.f;
+      for (int k = 0; k < cnt; ++k) {
+        const int ref = pr[tid * PS + k];
+        float acc = 0.f;
+        if (ref >= 0) {
+          const float4 *nb = ser + (size_t)ref * ss;
+#pragma unroll
+          for (int jj = 0; jj < kChunk / 2; ++jj) {
+            if (jj < s4) {
+              const float4 v = nb[jj];
+              acc += cen[jj].x * v.x + cen[jj].y * v.y;
+              if (2 * jj + 1 < nifg) acc += cen[jj].z * v.z + cen[jj].w * v.w;
+            }
+          }
+        } else {
+          const float4 *nb = zt + (size_t)(-ref - 1) * s4;
+#pragma unroll
+          for (int jj = 0; jj < kChunk / 2; ++jj) {
+            if (jj < s4) {
+              const float4 v = nb[jj];
+              acc += cen[jj].x * v.x + cen[jj].y * v.y;
+              if (2 * jj + 1 < nifg) acc += cen[jj].z * v.z + cen[jj].w * v.w;
+            }
+          }
+        }
+        const float sim = acc / fn;
+        if (best < sim) best = sim;
+      }
+      out[q] = cnt ? best : 0.f;
+    }
+    return;  // no barrier follows
+  }
+  // 4. phase B: the (centre, neighbour) similarities
+  for (int pidx = tid; pidx < nc * NN; pidx += 256) {
+    const int ci = pidx / NN, k = pidx - ci * NN;
+    if (k >= ccnt[ci]) continue;
+    const int q = cpix[ci];
+    const int ref = pr[ci * PS + k];
+    int cs = -1;  // the centre's slot (median pass: a staged candidate)
+    if (MEDIAN) {
+      const int r = q / ncol, c = q - r * ncol;
+      cs = smap[(r - r0) * ww + (c - c0)];
+    }
+    float sim;
+    if (ref >= 0) {
+      if (cs >= 0)
+        sim = ps_dot<true, true>(ser + (size_t)cs * ss, ser + (size_t)ref * ss, nifg, s4, fn);
+      else
+        sim = ps_dot<false, true>(zt + (size_t)q * s4, ser + (size_t)ref * ss, nifg, s4, fn);
+    } else {
+      const float4 *nb = zt + (size_t)(-ref - 1) * s4;
+      if (cs >= 0)
+        sim = ps_dot<true, false>(ser + (size_t)cs * ss, nb, nifg, s4, fn);
+      else
+        sim = ps_dot<false, false>(zt + (size_t)q * s4, nb, nifg, s4, fn);
+    }
+    pr[ci * PS + k] = __float_as_int(sim);
+  }
+  __syncthreads();
+  // 5. phase C: median or maximum per centre
+  if (tid < nc) {
+    const int cnt = ccnt[tid];
+    float res = 0.f;
+    if (MEDIAN) {
+      constexpr int H = NN / 2 + 1;
+      float srt[H];
+#pragma unroll
+      for (int h = 0; h < H; ++h) srt[h] = __builtin_inff();
+      for (int k = 0; k < cnt; ++k) {
+        float v = __int_as_float(pr[tid * PS + k]);
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+          const float lo = fminf(srt[h], v);
+          v = fmaxf(srt[h], v);
+          srt[h] = lo;
+        }
+      }
+      const int m = cnt / 2;
+#pragma unroll
+      for (int h = 0; h < H; ++h)
+        if (h == m) res = srt[h];
+      if (cnt == 0) res = 0.f;
+    } else {
+      float best = -1.f;
+      for (int k = 0; k < cnt; ++k) {
+        const float v = __int_as_float(pr[tid * PS + k]);
+        if (best < v) best = v;
+      }
+      res = cnt ? best : 0.f;
+    }
+    out[cpix[tid]] = res;
+  }
+}
+
 struct PsWork {
   float2 *zt;
   unsigned *bits;
@@ -519,6 +849,35 @@ int launch_sim_nn(s1ifg_ctx *ctx, const PsWork &wk, int nifg, int nrow, int ncol
   return S1IFG_OK;
 }
 
+// The tiled form for nmax <= 64 when at least 32 series fit in LDS;
+// *used = false leaves the pass to the list form.
+template <int NN, bool MEDIAN>
+int launch_tile_nn(s1ifg_ctx *ctx, const PsWork &wk, int nifg, int nrow, int ncol,
+                   const int32_t *tab, int noff, int extent, int nmax, float *out, bool *used) {
+  const int halo = std::min(extent, kHalo);
+  const int ww = kPT + 2 * halo;
+  const int s4 = wk.stride / 2;
+  const int static_lds = 2 * 256 * 4 + 64;
+  const int budget = (NN <= 32 ? kTileLds : 2 * kTileLds - 4096) - static_lds;
+  const int fixed = tile_fixed_lds<NN>(ww);
+  const int ss = s4 | 1;
+  const int cap = std::min(1024, (budget - fixed) / (ss * 16 + 4));
+  *used = false;
+  if (cap < 32) return S1IFG_OK;
+  const int dyn = fixed + cap * (ss * 16 + 4);
+  auto kern = k_ps_tile<NN, MEDIAN>;
+  S1_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+  const int ntiles = ((nrow + kPT - 1) / kPT) * ((ncol + kPT - 1) / kPT);
+  const int per_xcd = (ntiles + kXcds - 1) / kXcds;
+  hipLaunchKernelGGL(kern, dim3((unsigned)(per_xcd * kXcds)), dim3(256), dyn, ctx->stream,
+                     reinterpret_cast<const float4 *>(wk.zt), nifg, s4, nrow, ncol, wk.bits,
+                     reinterpret_cast<const int *>(tab), noff, nmax, halo, cap, per_xcd, out);
+  S1_LAUNCH_CHECK("k_ps_tile");
+  *used = true;
+  return S1IFG_OK;
+}
+
 template <bool MEDIAN>
 int launch_sim(s1ifg_ctx *ctx, const PsWork &wk, int nifg, int nrow, int ncol, const int32_t *tab,
                int noff, int nneigh, float *out) {
@@ -526,6 +885,23 @@ int launch_sim(s1ifg_ctx *ctx, const PsWork &wk, int nifg, int nrow, int ncol, c
   if (nmax <= 32) return launch_sim_nn<32, MEDIAN>(ctx, wk, nifg, nrow, ncol, tab, noff, nmax, out);
   if (nmax <= 64) return launch_sim_nn<64, MEDIAN>(ctx, wk, nifg, nrow, ncol, tab, noff, nmax, out);
   return launch_sim_nn<128, MEDIAN>(ctx, wk, nifg, nrow, ncol, tab, noff, nmax, out);
+}
+
+// One similarity pass over the pixels whose bit is set (median) or clear
+// (maximum): the tiled form where it applies, else list + k_ps_sim.
+template <bool MEDIAN>
+int run_pass(s1ifg_ctx *ctx, const PsWork &wk, int nifg, int nrow, int ncol, const int32_t *tab,
+             int noff, int extent, int nneigh, float *out) {
+  const int nmax = std::max(nneigh, 1);
+  bool used = false;
+  int rc = S1IFG_OK;
+  if (nmax <= 32)
+    rc = launch_tile_nn<32, MEDIAN>(ctx, wk, nifg, nrow, ncol, tab, noff, extent, nmax, out, &used);
+  else if (nmax <= 64)
+    rc = launch_tile_nn<64, MEDIAN>(ctx, wk, nifg, nrow, ncol, tab, noff, extent, nmax, out, &used);
+  if (rc || used) return rc;
+  if ((rc = launch_list(ctx, wk, MEDIAN ? 0 : 1))) return rc;
+  return launch_sim<MEDIAN>(ctx, wk, nifg, nrow, ncol, tab, noff, nneigh, out);
 }
 
 }  // namespace
@@ -583,13 +959,12 @@ int s1ifg_ps_median_similarity(s1ifg_ctx *ctx, const s1ifg_cf32 *stack_norm, int
   if (rc) return rc;
   if (!sim_median) return set_error(S1IFG_EINVAL, "null output");
   const int32_t *tab;
-  int noff;
+  int noff, extent;
   PsWork wk;
-  if ((rc = ps_table(ctx, rdmin, rdmax, &tab, &noff))) return rc;
+  if ((rc = ps_table(ctx, rdmin, rdmax, &tab, &noff, &extent))) return rc;
   if ((rc = ps_workspace(ctx, nifg, nrow * ncol, &wk))) return rc;
   if ((rc = launch_prep(ctx, wk, stack_norm, nifg, 0, ps, sim_median, 0, 0.f))) return rc;
-  if ((rc = launch_list(ctx, wk, 0))) return rc;
-  return launch_sim<true>(ctx, wk, nifg, nrow, ncol, tab, noff, nneigh, sim_median);
+  return run_pass<true>(ctx, wk, nifg, nrow, ncol, tab, noff, extent, nneigh, sim_median);
 }
 
 int s1ifg_ps_maximum_similarity(s1ifg_ctx *ctx, const s1ifg_cf32 *stack_norm, int nifg, int nrow,
@@ -599,13 +974,12 @@ int s1ifg_ps_maximum_similarity(s1ifg_ctx *ctx, const s1ifg_cf32 *stack_norm, in
   if (rc) return rc;
   if (!sim_max) return set_error(S1IFG_EINVAL, "null output");
   const int32_t *tab;
-  int noff;
+  int noff, extent;
   PsWork wk;
-  if ((rc = ps_table(ctx, rdmin, rdmax, &tab, &noff))) return rc;
+  if ((rc = ps_table(ctx, rdmin, rdmax, &tab, &noff, &extent))) return rc;
   if ((rc = ps_workspace(ctx, nifg, nrow * ncol, &wk))) return rc;
   if ((rc = launch_prep(ctx, wk, stack_norm, nifg, 0, ps, sim_max, 1, 1.f))) return rc;
-  if ((rc = launch_list(ctx, wk, 1))) return rc;
-  return launch_sim<false>(ctx, wk, nifg, nrow, ncol, tab, noff, nneigh, sim_max);
+  return run_pass<false>(ctx, wk, nifg, nrow, ncol, tab, noff, extent, nneigh, sim_max);
 }
 
 int s1ifg_ps_similarity(s1ifg_ctx *ctx, const s1ifg_cf32 *stack, int nifg, int nrow, int ncol,
@@ -615,17 +989,16 @@ int s1ifg_ps_similarity(s1ifg_ctx *ctx, const s1ifg_cf32 *stack, int nifg, int n
   if (rc) return rc;
   if (!med_sim || !max_sim) return set_error(S1IFG_EINVAL, "null output");
   const int32_t *tab;
-  int noff;
+  int noff, extent;
   PsWork wk;
-  if ((rc = ps_table(ctx, rdmin, rdmax, &tab, &noff))) return rc;
+  if ((rc = ps_table(ctx, rdmin, rdmax, &tab, &noff, &extent))) return rc;
   if ((rc = ps_workspace(ctx, nifg, nrow * ncol, &wk))) return rc;
   if ((rc = launch_prep(ctx, wk, stack, nifg, 1, ps, med_sim, 0, 0.f))) return rc;
-  if ((rc = launch_list(ctx, wk, 0))) return rc;
-  if ((rc = launch_sim<true>(ctx, wk, nifg, nrow, ncol, tab, noff, nneigh, med_sim))) return rc;
+  if ((rc = run_pass<true>(ctx, wk, nifg, nrow, ncol, tab, noff, extent, nneigh, med_sim)))
+    return rc;
   hipLaunchKernelGGL(k_ps_update_bits, dim3((unsigned)((wk.ntiles + 3) / 4)), dim3(256), 0,
                      ctx->stream, reinterpret_cast<int *>(ps), med_sim, med_sim_th, wk.npix, wk.bits,
                      wk.counts, max_sim);
   S1_LAUNCH_CHECK("k_ps_update_bits");
-  if ((rc = launch_list(ctx, wk, 1))) return rc;
-  return launch_sim<false>(ctx, wk, nifg, nrow, ncol, tab, noff, nneigh, max_sim);
+  return run_pass<false>(ctx, wk, nifg, nrow, ncol, tab, noff, extent, nneigh, max_sim);
 }
