@@ -75,6 +75,8 @@ CASES = [
     (5, 64, 64, 0, 1, 4, 0.1),
     (8, 40, 40, 128, 0, 16, 0.25),
     (65, 23, 29, 10, 0, 5, 0.4),
+    (12, 70, 90, 50, 0, 9, 0.2),    # tiled, 64 neighbours, series in registers
+    (19, 48, 80, 32, 0, 16, 0.3),   # tiled, offsets beyond the staged halo
 ]
 
 
@@ -102,6 +104,18 @@ def test_masks_all_and_none(ctx, C):
     # threshold 0: every median (0 included) passes, every pixel is PS
     got = _gpu(ctx, z, np.zeros((33, 47), np.int32), 9, 0, 5, 0.0)
     assert got[2].all() and np.all(got[1] == 1)
+
+
+def test_tile_capacity_overflow(ctx, C):
+    """Every pixel a candidate with 19 interferograms and rdmax 16: a tile's
+    window holds more eligible series than its LDS slots, so part of them
+    (and the neighbours past the halo) are read from global memory; the
+    result must not change."""
+    from sentinel_gpu_processor_amd import synthetic as S
+    z, _, _ = S.ps_stack(19, 60, 70, seed=21)
+    for ps, th in ((np.ones((60, 70), np.int32), 0.3), (np.ones((60, 70), np.int32), -2.0)):
+        got = _gpu(ctx, z, ps, 32, 0, 16, th)
+        _same(got, C.ps_similarity(z, ps, 32, 0, 16, th))
 
 
 def test_stage_entry_points(ctx, C):
