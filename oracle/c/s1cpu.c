@@ -435,11 +435,15 @@ void s1c_goldstein(const float *in, int nrow, int ncol, float alpha, float *out)
   }
   float w1[GN];
   for (int t = 0; t < GN; ++t) w1[t] = (float)(t < GN / 2 ? t : GN - 1 - t);
-  /* patch rows p and p + 4 do not overlap: four phases, each parallel over
-   * its patch rows, add in a fixed order */
-  for (int phase = 0; phase < 4; ++phase) {
+  /* patch rows p and p + 4 do not overlap, except that the last one may be
+   * clamped back onto its predecessors (goldstein.cu:308-313): four phases,
+   * each parallel over its unclamped patch rows, then a clamped last row on
+   * its own */
+  for (int phase = 0; phase < 5; ++phase) {
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int pi = phase; pi < pi_n; pi += 4) {
+    for (int pi = phase < 4 ? phase : 0; pi < pi_n; pi += phase < 4 ? 4 : 1) {
+      const int clamped = pi * inc > nrow - GN;
+      if (clamped != (phase == 4)) continue;
       const int oy = pi * inc < nrow - GN ? pi * inc : nrow - GN;
       float re[GN * GN], im[GN * GN], m[GN * GN];
       for (int pj = 0; pj < pj_n; ++pj) {
