@@ -25,12 +25,24 @@
 // offset, which carries over to bursts whose ids match their predecessor's,
 // or whose overlap is empty, as in the reference.
 //
+// When no burst's rows reach past its predecessor's into an earlier burst's
+// (the usual layout of a subswath: each burst overlaps the next one only),
+// the pixels a burst meets in the mosaic are its predecessor's, rotated by
+// the predecessor's offset.  Then every boundary's estimate is taken
+// against the predecessor's own pixels, all boundaries in the same launches
+// (grid y = boundary): the angle of the mean difference shifts by the
+// predecessor's offset, the residual angles do not, so k_ov_chain later
+// chains the offsets in one thread (the carried offsets included) and the
+// bursts are applied in order with them.  Otherwise the boundaries are
+// estimated one after the other against the mosaic.
+//
 // Everything stays on the device (no host round trip per burst).  Partials
 // are summed in a fixed order and histograms are integer counts, so results
 // do not depend on timing.
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <vector>
 
 #include "s1ifg_internal.h"
 
@@ -61,6 +73,8 @@ struct Sel {
 struct StitchState {
   float off;            // running mean_phase_diff (0 before the first estimate)
   float m;              // angle of the mean phase difference
+  float mr, mi;         // the mean phase difference (complex64)
+  float med;            // median of the residual angles
   long long n;          // overlap count of the current burst
   int rmin, rmax;       // overlap rows (burst window rows)
   Sel sel[3];           // sel[p]: the state histogram pass p filters with
@@ -78,6 +92,45 @@ struct Win {
   long ld;           // mosaic row stride (elements)
   int o0, o1;        // burst rows [o0, o1) inside earlier bursts' boxes
 };
+
+// Where a boundary estimate reads the "old" pixels: the mosaic, or the
+// predecessor burst (origin r0, c0 in the mosaic, h x w, row stride ld).
+template <typename T>
+struct Old {
+  const T *base;
+  long ld;
+  int r0, c0, h, w;
+};
+
+// One boundary of a launch over several (blockIdx.y).
+template <typename T>
+struct Est {
+  const T *b;        // the new burst
+  Win w;
+  Old<T> old;
+  Workspace *ws;
+  unsigned *keys;    // (o1 - o0) x w keys
+};
+constexpr int kMaxEst = 24;  // boundaries per launch (kernel arguments <= 4 KB)
+template <typename T>
+struct EstBatch {
+  Est<T> e[kMaxEst];
+};
+
+// Row r of the new burst's window against the old pixels: pointer to the old
+// row at the window's first column, and the window columns [lo, hi) the old
+// source covers (everything for the mosaic).
+template <typename T>
+__device__ __forceinline__ const T *old_row(const Est<T> &e, int r, int &lo, int &hi) {
+  const int R = e.w.r0 + r - e.old.r0;
+  if (R < 0 || R >= e.old.h) {
+    lo = hi = 0;
+    return e.old.base;
+  }
+  lo = max(0, e.old.c0 - e.w.c0);
+  hi = min(e.w.w, e.old.c0 + e.old.w - e.w.c0);
+  return e.old.base + (long)R * e.old.ld + (e.w.c0 - e.old.c0);
+}
 
 __device__ __forceinline__ bool has_data(float2 v) { return v.x != 0.f; }
 __device__ __forceinline__ bool has_data(float v) { return v != 0.f; }
@@ -179,10 +232,12 @@ __device__ __forceinline__ void block_sums(BlockSums &S) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_ov_partial(const T *__restrict__ b, Win w,
-                                                    const T *__restrict__ mosaic,
-                                                    Workspace *__restrict__ ws) {
+__global__ __launch_bounds__(256) void k_ov_partial(const EstBatch<T> B) {
   __shared__ BlockSums S;
+  const Est<T> &e = B.e[blockIdx.y];
+  const T *__restrict__ b = e.b;
+  const Win w = e.w;
+  Workspace *__restrict__ ws = e.ws;
   const int t = threadIdx.x;
   double sr = 0.0, si = 0.0;
   long long n = 0;
@@ -194,13 +249,14 @@ __global__ __launch_bounds__(256) void k_ov_partial(const T *__restrict__ b, Win
     const int r = w.o0 + rr;
     const int cb = (int)(it - (long)rr * nchunk) * kChunk, ce = min(cb + kChunk, w.w);
     const T *brow = b + (long)r * w.w;
-    const T *mrow = mosaic + (long)(w.r0 + r) * w.ld + w.c0;
+    int lo, hi;
+    const T *mrow = old_row(e, r, lo, hi);
     bool any = false;
     T o[4], v[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int c = cb + t + 256 * k;
-      o[k] = c < ce ? mrow[c] : T{};
+      o[k] = c < ce && c >= lo && c < hi ? mrow[c] : T{};
       v[k] = c < ce ? brow[c] : T{};
     }
 #pragma unroll
@@ -232,7 +288,7 @@ __global__ __launch_bounds__(256) void k_ov_partial(const T *__restrict__ b, Win
 // np.angle of the complex64 mean) and the overlap rows.
 struct Overlap {
   long long n;
-  float m;
+  float m, mr, mi;
   int rmin, rmax;
 };
 
@@ -258,10 +314,11 @@ __device__ __forceinline__ Overlap sum_partials(const Workspace *ws, BlockSums &
   block_sums(S);
   Overlap o;
   o.n = S.n[0];
-  o.m = 0.f;
+  o.m = o.mr = o.mi = 0.f;
   if (o.n > 0) {
-    const float mr = (float)(S.re[0] / (double)o.n), mi = (float)(S.im[0] / (double)o.n);
-    o.m = atan2f(mi, mr);
+    o.mr = (float)(S.re[0] / (double)o.n);
+    o.mi = (float)(S.im[0] / (double)o.n);
+    o.m = atan2f(o.mi, o.mr);
   }
   o.rmin = S.lo[0];
   o.rmax = S.hi[0];
@@ -334,11 +391,13 @@ constexpr int kShift[3] = {21, 10, 0}, kWidth[3] = {11, 11, 10};
 // by the digit selection over pass p-1's histogram; block 0 stores it (and
 // pass 0 the overlap figures) for the following kernels.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ov_hist(const T *__restrict__ b, Win w,
-                                                 const T *__restrict__ mosaic,
-                                                 Workspace *__restrict__ ws, int pass,
-                                                 unsigned *__restrict__ keys) {
+__global__ __launch_bounds__(256) void k_ov_hist(const EstBatch<T> B, int pass) {
   __shared__ unsigned h[2][kBins];
+  const Est<T> &e = B.e[blockIdx.y];
+  const T *__restrict__ b = e.b;
+  const Win w = e.w;
+  Workspace *__restrict__ ws = e.ws;
+  unsigned *__restrict__ keys = e.keys;
   __shared__ union {
     BlockSums S;
     SelScratch X;
@@ -362,6 +421,8 @@ __global__ __launch_bounds__(256) void k_ov_hist(const T *__restrict__ b, Win w,
       gst.n = cnt;
       if (cnt > 0) {
         gst.m = m;
+        gst.mr = o.mr;
+        gst.mi = o.mi;
         gst.rmin = rmin;
         gst.rmax = rmax;
         gst.sel[0] = sel;
@@ -399,12 +460,13 @@ __global__ __launch_bounds__(256) void k_ov_hist(const T *__restrict__ b, Win w,
       // keys of the overlap cells (0 elsewhere: no finite angle maps to 0),
       // kept for the later passes
       const T *brow = b + (long)r * w.w;
-      const T *mrow = mosaic + (long)(w.r0 + r) * w.ld + w.c0;
+      int lo, hi;
+      const T *mrow = old_row(e, r, lo, hi);
       T o[4], v[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int c = cb + t + 256 * k;
-        o[k] = c < ce ? mrow[c] : T{};
+        o[k] = c < ce && c >= lo && c < hi ? mrow[c] : T{};
         v[k] = c < ce ? brow[c] : T{};
       }
 #pragma unroll
@@ -440,8 +502,10 @@ __global__ __launch_bounds__(256) void k_ov_hist(const T *__restrict__ b, Win w,
 // One block: the digit selection over the last pass's histogram gives both
 // order statistics; the running offset becomes m + median.  All histograms
 // are cleared for the next boundary.
-__global__ __launch_bounds__(256) void k_ov_median(Workspace *__restrict__ ws) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_ov_median(const EstBatch<T> B, int chained) {
   __shared__ SelScratch X;
+  Workspace *__restrict__ ws = B.e[blockIdx.x].ws;
   StitchState &gst = ws->st;
   const long long cnt = gst.n;
   if (cnt == 0) return;
@@ -450,7 +514,8 @@ __global__ __launch_bounds__(256) void k_ov_median(Workspace *__restrict__ ws) {
   if (threadIdx.x == 0) {
     const float a = key_value(sel.prefix[0]), bb = key_value(sel.prefix[1]);
     const float med = (cnt & 1) ? a : __fmul_rn(__fadd_rn(a, bb), 0.5f);
-    gst.off = __fadd_rn(gst.m, med);
+    gst.med = med;
+    if (!chained) gst.off = __fadd_rn(gst.m, med);
   }
 }
 
@@ -475,10 +540,9 @@ __device__ __forceinline__ float corrected(float v, float off) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_apply(const T *__restrict__ b, Win w,
-                                               T *__restrict__ mosaic,
-                                               const Workspace *__restrict__ ws,
+                                               T *__restrict__ mosaic, const float *offp,
                                                double *__restrict__ offsets, int burst) {
-  const float off = ws->st.off;
+  const float off = *offp;
   const int r = blockIdx.x;
   const T *brow = b + (long)r * w.w;
   T *mrow = mosaic + (long)(w.r0 + r) * w.ld + w.c0;
@@ -507,10 +571,9 @@ __global__ __launch_bounds__(256) void k_apply(const T *__restrict__ b, Win w,
 // k_apply for complex bursts with 16-byte alignment (even window width,
 // column offset and row stride): two cells per 16-byte access.
 __global__ __launch_bounds__(256) void k_apply_c2(const float4 *__restrict__ b, Win w,
-                                                  float2 *__restrict__ mosaic,
-                                                  const Workspace *__restrict__ ws,
+                                                  float2 *__restrict__ mosaic, const float *offp,
                                                   double *__restrict__ offsets, int burst) {
-  const float off = ws->st.off;
+  const float off = *offp;
   const int r = blockIdx.x;
   const int w2 = w.w >> 1;
   const float4 *brow = b + (long)r * w2;
@@ -564,43 +627,123 @@ __global__ __launch_bounds__(256) void k_merge(const T *__restrict__ src, int h,
   }
 }
 
+// Clears the state and histograms of workspaces ws[0 .. gridDim.x).
 __global__ void k_init_state(Workspace *__restrict__ ws, double *__restrict__ offsets) {
+  Workspace *w = ws + blockIdx.x;
   if (threadIdx.x == 0) {
-    ws->st.off = 0.f;
-    ws->st.n = 0;
-    if (offsets) offsets[0] = 0.0;
+    w->st.off = 0.f;
+    w->st.n = 0;
+    if (offsets && blockIdx.x == 0) offsets[0] = 0.0;
   }
-  for (int k = threadIdx.x; k < 3 * 2 * kBins; k += blockDim.x) (&ws->hist[0][0][0])[k] = 0u;
+  for (int k = threadIdx.x; k < 3 * 2 * kBins; k += blockDim.x) (&w->hist[0][0][0])[k] = 0u;
+}
+
+// The running offsets of the bursts from the boundary estimates taken
+// against each predecessor's own pixels (one thread): at an estimated
+// boundary with overlap, the mean difference is rotated by the
+// predecessor's offset (the mosaic holds the predecessor rotated by it)
+// and the residual median added (interfere.py:121-133); elsewhere the
+// offset carries over.  est[i]: the workspace of burst i's boundary, or -1.
+__device__ void ov_chain(const Workspace *__restrict__ ws, const int *est, int n,
+                         float *__restrict__ offs) {
+  float off = 0.f;
+  offs[0] = 0.f;
+  for (int i = 1; i < n; ++i) {
+    const int e = est[i];
+    if (e >= 0 && ws[e].st.n > 0) {
+      const StitchState &st = ws[e].st;
+      float sn, cs;
+      sincosf(off, &sn, &cs);
+      const float2 mean = cmul(make_float2(st.mr, st.mi), make_float2(cs, sn));
+      off = __fadd_rn(atan2f(mean.y, mean.x), st.med);
+    }
+    offs[i] = off;
+  }
+}
+
+constexpr int kMaxChain = 256;  // bursts of the chained form (k_ov_chain's argument)
+struct ChainArgs {
+  int est[kMaxChain];
+};
+
+__global__ void k_ov_chain(const Workspace *__restrict__ ws, const ChainArgs A, int n,
+                           float *__restrict__ offs) {
+  if (threadIdx.x == 0) ov_chain(ws, A.est, n, offs);
+}
+
+template <typename T>
+int launch_estimates(hipStream_t s, const std::vector<Est<T>> &es, int chained) {
+  for (size_t i0 = 0; i0 < es.size(); i0 += kMaxEst) {
+    const int nb = (int)std::min(es.size() - i0, (size_t)kMaxEst);
+    EstBatch<T> B{};
+    for (int k = 0; k < nb; ++k) B.e[k] = es[i0 + k];
+    hipLaunchKernelGGL(k_ov_partial<T>, dim3(kOvBlocks, nb), dim3(256), 0, s, B);
+    S1_LAUNCH_CHECK("k_ov_partial");
+    for (int pss = 0; pss < 3; ++pss) {
+      hipLaunchKernelGGL(k_ov_hist<T>, dim3(kHistBlocks, nb), dim3(256), 0, s, B, pss);
+      S1_LAUNCH_CHECK("k_ov_hist");
+    }
+    hipLaunchKernelGGL(k_ov_median<T>, dim3(nb), dim3(256), 0, s, B, chained);
+    S1_LAUNCH_CHECK("k_ov_median");
+  }
+  return S1IFG_OK;
 }
 
 template <typename T>
 int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosaic, int left,
                 int top, int right, int bottom, double *offsets) {
   const int mh = bottom - top, mw = right - left;
-  // key buffer: one u32 per cell of the rows a burst shares with earlier
-  // bursts' boxes (the largest such band over the bursts)
-  size_t nkeys = 0;
+  const s1ifg_stitch_burst &b0 = bursts[0];
+  // rows [o0, o1) of each burst inside earlier bursts' boxes (burst 0's box
+  // clipped to the bounds), and whether every burst meets only its
+  // predecessor among them (the chained form)
+  std::vector<int> o0(n, 0), o1(n, 0), est(n, -1);
+  bool chained = n <= kMaxChain;
+  int nest = 0;
   {
-    int c0 = max(bursts[0].top, top), c1 = min(bursts[0].bottom, bottom);
+    int c0 = max(b0.top, top), c1 = min(b0.bottom, bottom);
     for (int i = 1; i < n; ++i) {
       const s1ifg_stitch_burst &bi = bursts[i];
       const int h = bi.bottom - bi.top;
-      const int o0 = min(max(c0 - bi.top, 0), h), o1 = max(min(c1 - bi.top, h), o0);
-      if (bi.estimate) nkeys = std::max(nkeys, (size_t)(o1 - o0) * (size_t)(bi.right - bi.left));
+      o0[i] = min(max(c0 - bi.top, 0), h);
+      o1[i] = max(min(c1 - bi.top, h), o0[i]);
+      for (int j = 0; j + 1 < i && chained; ++j) {
+        const int tj = j == 0 ? max(b0.top, top) : bursts[j].top;
+        const int bj = j == 0 ? min(b0.bottom, bottom) : bursts[j].bottom;
+        if (tj < bi.bottom && bi.top < bj) chained = false;
+      }
+      if (bi.estimate) est[i] = nest++;
       c0 = min(c0, bi.top);
       c1 = max(c1, bi.bottom);
     }
   }
-  const size_t ws_head = (sizeof(Workspace) + 255) & ~(size_t)255;
+  // workspace: states and histograms (one per estimated boundary when
+  // chained, else one), the offsets, the keys (per boundary when chained,
+  // else the largest band)
+  const int nws = chained ? std::max(nest, 1) : 1;
+  std::vector<size_t> koff(n, 0);
+  size_t nkeys = 0;
+  for (int i = 1; i < n; ++i) {
+    if (est[i] < 0) continue;
+    const size_t band = (size_t)(o1[i] - o0[i]) * (size_t)(bursts[i].right - bursts[i].left);
+    if (chained) {
+      koff[i] = nkeys;
+      nkeys += band;
+    } else {
+      nkeys = std::max(nkeys, band);
+    }
+  }
+  const size_t ws_bytes = ((size_t)nws * sizeof(Workspace) + 255) & ~(size_t)255;
+  const size_t off_bytes = ((size_t)n * sizeof(float) + 255) & ~(size_t)255;
   void *p = nullptr;
-  int rc = ctx_workspace(ctx, ws_head + nkeys * sizeof(unsigned), &p);
+  int rc = ctx_workspace(ctx, ws_bytes + off_bytes + nkeys * sizeof(unsigned), &p);
   if (rc) return rc;
   Workspace *ws = (Workspace *)p;
-  unsigned *keys = (unsigned *)((char *)p + ws_head);
+  float *offs = (float *)((char *)p + ws_bytes);
+  unsigned *keys = (unsigned *)((char *)p + ws_bytes + off_bytes);
   hipStream_t s = ctx->stream;
-  hipLaunchKernelGGL(k_init_state, dim3(1), dim3(256), 0, s, ws, offsets);
+  hipLaunchKernelGGL(k_init_state, dim3(nws), dim3(256), 0, s, ws, offsets);
   S1_LAUNCH_CHECK("k_init_state");
-  const s1ifg_stitch_burst &b0 = bursts[0];
   const int bw0 = b0.right - b0.left, bc0 = b0.left - left;
   if (sizeof(T) == 8 && mw % 2 == 0 && bw0 % 2 == 0 && bc0 % 2 == 0 &&
       (uintptr_t)b0.data % 16 == 0 && (uintptr_t)mosaic % 16 == 0) {
@@ -613,41 +756,53 @@ int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosa
                        b0.bottom - b0.top, bw0, b0.top - top, bc0, mosaic, mh, mw);
     S1_LAUNCH_CHECK("k_place_first");
   }
-  // rows of the mosaic that earlier bursts may have written (burst 0's box
-  // clipped to the bounds, then the later bursts' boxes)
-  int cov0 = max(b0.top, top), cov1 = min(b0.bottom, bottom);
+  auto window = [&](int i) {
+    const s1ifg_stitch_burst &bi = bursts[i];
+    return Win{bi.bottom - bi.top, bi.right - bi.left, bi.top - top, bi.left - left, (long)mw,
+               o0[i], o1[i]};
+  };
+  if (chained && nest > 0) {
+    // every boundary against its predecessor's pixels, in one set of
+    // launches; then the offsets in one thread
+    std::vector<Est<T>> es;
+    for (int i = 1; i < n; ++i) {
+      if (est[i] < 0) continue;
+      const s1ifg_stitch_burst &bp = bursts[i - 1];
+      Old<T> old{(const T *)bp.data, (long)(bp.right - bp.left), bp.top - top, bp.left - left,
+                 bp.bottom - bp.top, bp.right - bp.left};
+      es.push_back(Est<T>{(const T *)bursts[i].data, window(i), old, ws + est[i], keys + koff[i]});
+    }
+    if ((rc = launch_estimates<T>(s, es, 1))) return rc;
+    ChainArgs A{};
+    for (int i = 0; i < n; ++i) A.est[i] = est[i];
+    hipLaunchKernelGGL(k_ov_chain, dim3(1), dim3(64), 0, s, (const Workspace *)ws, A, n, offs);
+    S1_LAUNCH_CHECK("k_ov_chain");
+  }
   for (int i = 1; i < n; ++i) {
     const s1ifg_stitch_burst &bi = bursts[i];
     const T *bd = (const T *)bi.data;
-    const int h = bi.bottom - bi.top;
-    Win w{h, bi.right - bi.left, bi.top - top, bi.left - left, (long)mw, 0, 0};
-    w.o0 = min(max(cov0 - bi.top, 0), h);
-    w.o1 = max(min(cov1 - bi.top, h), w.o0);
-    if (bi.estimate) {
-      hipLaunchKernelGGL(k_ov_partial<T>, dim3(kOvBlocks), dim3(256), 0, s, bd, w,
-                         (const T *)mosaic, ws);
-      S1_LAUNCH_CHECK("k_ov_partial");
-      for (int pss = 0; pss < 3; ++pss) {
-        hipLaunchKernelGGL(k_ov_hist<T>, dim3(kHistBlocks), dim3(256), 0, s, bd, w,
-                           (const T *)mosaic, ws, pss, keys);
-        S1_LAUNCH_CHECK("k_ov_hist");
+    const Win w = window(i);
+    const float *offp = offs + i;
+    if (!chained) {
+      if (bi.estimate) {
+        Old<T> old{(const T *)mosaic, (long)mw, 0, 0, mh, mw};
+        std::vector<Est<T>> es{Est<T>{bd, w, old, ws, keys}};
+        if ((rc = launch_estimates<T>(s, es, 0))) return rc;
       }
-      hipLaunchKernelGGL(k_ov_median, dim3(1), dim3(256), 0, s, ws);
-      S1_LAUNCH_CHECK("k_ov_median");
+      offp = &ws->st.off;
+    } else if (nest == 0) {
+      offp = &ws->st.off;  // 0: nothing estimated
     }
     const bool pairs = sizeof(T) == 8 && w.w % 2 == 0 && w.c0 % 2 == 0 && w.ld % 2 == 0 &&
                        (uintptr_t)bd % 16 == 0 && (uintptr_t)mosaic % 16 == 0;
     if (pairs) {
-      hipLaunchKernelGGL(k_apply_c2, dim3(h), dim3(256), 0, s, (const float4 *)bd, w,
-                         (float2 *)mosaic, (const Workspace *)ws, offsets, i);
+      hipLaunchKernelGGL(k_apply_c2, dim3(w.h), dim3(256), 0, s, (const float4 *)bd, w,
+                         (float2 *)mosaic, offp, offsets, i);
       S1_LAUNCH_CHECK("k_apply_c2");
     } else {
-      hipLaunchKernelGGL(k_apply<T>, dim3(h), dim3(256), 0, s, bd, w, mosaic,
-                         (const Workspace *)ws, offsets, i);
+      hipLaunchKernelGGL(k_apply<T>, dim3(w.h), dim3(256), 0, s, bd, w, mosaic, offp, offsets, i);
       S1_LAUNCH_CHECK("k_apply");
     }
-    cov0 = min(cov0, bi.top);
-    cov1 = max(cov1, bi.bottom);
   }
   return S1IFG_OK;
 }

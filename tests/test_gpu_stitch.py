@@ -88,6 +88,20 @@ def test_stitch_carry_empty_and_ragged(out_float):
 
 
 @pytest.mark.parametrize("out_float", [False, True])
+def test_stitch_three_bursts_on_a_row(out_float):
+    """Burst 2 reaches rows of burst 0 as well as burst 1's: the boundaries
+    are then estimated one after the other against the mosaic (the other
+    tests have each burst meet only its predecessor, which lets all
+    boundaries be estimated at once)."""
+    rng = np.random.default_rng(15)
+    spans = [(0, 100), (40, 140), (90, 200), (185, 260)]
+    offs = [0.2, 1.3, -2.0, 0.7]
+    ids = [("t", str(k)) for k in range(4)]
+    bursts, lst = _make(rng, 240, spans, offs, ids, out_float)
+    _check(bursts, lst, ids, out_float)
+
+
+@pytest.mark.parametrize("out_float", [False, True])
 def test_stitch_even_offsets(out_float):
     """Even mosaic width with even, non-zero burst column offsets: the
     16-byte complex kernels (placement of burst 0 at column 2 of the box,
