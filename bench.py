@@ -563,10 +563,10 @@ def run_stitch(args):
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / steps
     P, M, O = h * w, mos.numel(), ov * w
-    # compulsory traffic: burst 0 read and the mosaic written once; per later
-    # burst its pixels read and its new pixels (all but the overlap) written,
-    # and the overlap's mosaic and burst pixels read for the offset estimate
-    alg = 8 * (P + M + (n - 1) * (2 * P + O))
+    # compulsory traffic: every burst read once and the mosaic written once;
+    # per boundary the overlap rows of both bursts read once more for the
+    # offset estimate (it has to finish before the mosaic can be composed)
+    alg = 8 * (n * P + M + (n - 1) * 2 * O)
     gbs = alg / (ms * 1e-3) / 1e9
     r = dict(workload=f"{n} bursts of {h} x {w} complex64 look cells, {ov}-row overlaps, "
                       f"every boundary estimated",

@@ -617,6 +617,94 @@ __global__ __launch_bounds__(256) void k_apply_c2(const float4 *__restrict__ b, 
   if (offsets && r == 0 && threadIdx.x == 0) offsets[burst] = (double)off;
 }
 
+// The whole mosaic in one pass once the offsets are known (the chained
+// form), with the result of placing burst 0 and applying the bursts in
+// order (interfere.py:97-143): a pixel starts as burst 0's value where its
+// box covers it, else 0; each later burst, in order, replaces it by its own
+// value rotated / shifted by its offset where that value has data and the
+// pixel has none.  One mosaic row per block.  In the chained form at most
+// two bursts cover a row (bursts that share rows are consecutive); a row
+// covered by more takes the general loop.
+constexpr int kMaxCompose = 96;
+template <typename T>
+struct ComposeArgs {
+  struct B {
+    const T *d;
+    int top, bottom, left, right;
+  } b[kMaxCompose];
+};
+
+template <typename T>
+__device__ __forceinline__ void compose_step(T &cur, T x, bool first, float off) {
+  if (first)
+    cur = x;
+  else if (!has_data(cur) && has_data(x))
+    cur = off != 0.f ? corrected(x, off) : x;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_compose(const ComposeArgs<T> A, int n, int top0, int left0,
+                                                 int mw, const float *__restrict__ offs,
+                                                 T *__restrict__ mosaic,
+                                                 double *__restrict__ offsets) {
+  const int R = blockIdx.x, gr = top0 + R;
+  int cv[2] = {-1, -1}, nc = 0;  // the bursts covering this row, in order
+  for (int j = 0; j < n && nc <= 2; ++j)
+    if (A.b[j].top <= gr && gr < A.b[j].bottom) {
+      if (nc < 2) cv[nc] = j;
+      ++nc;
+    }
+  T *mrow = mosaic + (long)R * mw;
+  if (nc > 2) {
+    for (int c = threadIdx.x; c < mw; c += 256) {
+      const int gc = left0 + c;
+      T cur{};
+      for (int j = 0; j < n; ++j) {
+        const auto &b = A.b[j];
+        if (b.top <= gr && gr < b.bottom && b.left <= gc && gc < b.right)
+          compose_step(cur, b.d[(long)(gr - b.top) * (b.right - b.left) + (gc - b.left)], j == 0,
+                       j ? offs[j] : 0.f);
+      }
+      mrow[c] = cur;
+    }
+  } else {
+    const T *row[2] = {nullptr, nullptr};
+    int l[2] = {0, 0}, r[2] = {0, 0};
+    float off[2] = {0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (cv[k] >= 0) {
+        const auto &b = A.b[cv[k]];
+        row[k] = b.d + (long)(gr - b.top) * (b.right - b.left) - b.left;
+        l[k] = b.left - left0;
+        r[k] = b.right - left0;
+        off[k] = cv[k] ? offs[cv[k]] : 0.f;
+      }
+    constexpr int kU = 4;  // cells in flight per thread and source
+    for (int c0 = threadIdx.x; c0 < mw; c0 += 256 * kU) {
+      T x[2][kU];
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int c = c0 + 256 * u;
+          x[k][u] = c >= l[k] && c < r[k] && c < mw ? row[k][left0 + c] : T{};
+        }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int c = c0 + 256 * u;
+        T cur{};
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+          if (c >= l[k] && c < r[k]) compose_step(cur, x[k][u], cv[k] == 0, off[k]);
+        if (c < mw) mrow[c] = cur;
+      }
+    }
+  }
+  if (offsets && R == 0)
+    for (int j = 1 + threadIdx.x; j < n; j += 256) offsets[j] = (double)offs[j];
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_merge(const T *__restrict__ src, int h, int w,
                                                T *__restrict__ dst, long ld) {
@@ -744,6 +832,39 @@ int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosa
   hipStream_t s = ctx->stream;
   hipLaunchKernelGGL(k_init_state, dim3(nws), dim3(256), 0, s, ws, offsets);
   S1_LAUNCH_CHECK("k_init_state");
+  auto window = [&](int i) {
+    const s1ifg_stitch_burst &bi = bursts[i];
+    return Win{bi.bottom - bi.top, bi.right - bi.left, bi.top - top, bi.left - left, (long)mw,
+               o0[i], o1[i]};
+  };
+  if (chained && n <= kMaxCompose) {
+    // every boundary against its predecessor's pixels, in one set of
+    // launches; the offsets in one thread; the mosaic in one pass
+    if (nest > 0) {
+      std::vector<Est<T>> es;
+      for (int i = 1; i < n; ++i) {
+        if (est[i] < 0) continue;
+        const s1ifg_stitch_burst &bp = bursts[i - 1];
+        Old<T> old{(const T *)bp.data, (long)(bp.right - bp.left), bp.top - top, bp.left - left,
+                   bp.bottom - bp.top, bp.right - bp.left};
+        es.push_back(
+            Est<T>{(const T *)bursts[i].data, window(i), old, ws + est[i], keys + koff[i]});
+      }
+      if ((rc = launch_estimates<T>(s, es, 1))) return rc;
+    }
+    ChainArgs A{};
+    for (int i = 0; i < n; ++i) A.est[i] = est[i];
+    hipLaunchKernelGGL(k_ov_chain, dim3(1), dim3(64), 0, s, (const Workspace *)ws, A, n, offs);
+    S1_LAUNCH_CHECK("k_ov_chain");
+    ComposeArgs<T> C{};
+    for (int i = 0; i < n; ++i)
+      C.b[i] = {(const T *)bursts[i].data, bursts[i].top, bursts[i].bottom, bursts[i].left,
+                bursts[i].right};
+    hipLaunchKernelGGL(k_compose<T>, dim3(mh), dim3(256), 0, s, C, n, top, left, mw,
+                       (const float *)offs, mosaic, offsets);
+    S1_LAUNCH_CHECK("k_compose");
+    return S1IFG_OK;
+  }
   const int bw0 = b0.right - b0.left, bc0 = b0.left - left;
   if (sizeof(T) == 8 && mw % 2 == 0 && bw0 % 2 == 0 && bc0 % 2 == 0 &&
       (uintptr_t)b0.data % 16 == 0 && (uintptr_t)mosaic % 16 == 0) {
@@ -756,43 +877,17 @@ int stitch_impl(s1ifg_ctx *ctx, const s1ifg_stitch_burst *bursts, int n, T *mosa
                        b0.bottom - b0.top, bw0, b0.top - top, bc0, mosaic, mh, mw);
     S1_LAUNCH_CHECK("k_place_first");
   }
-  auto window = [&](int i) {
-    const s1ifg_stitch_burst &bi = bursts[i];
-    return Win{bi.bottom - bi.top, bi.right - bi.left, bi.top - top, bi.left - left, (long)mw,
-               o0[i], o1[i]};
-  };
-  if (chained && nest > 0) {
-    // every boundary against its predecessor's pixels, in one set of
-    // launches; then the offsets in one thread
-    std::vector<Est<T>> es;
-    for (int i = 1; i < n; ++i) {
-      if (est[i] < 0) continue;
-      const s1ifg_stitch_burst &bp = bursts[i - 1];
-      Old<T> old{(const T *)bp.data, (long)(bp.right - bp.left), bp.top - top, bp.left - left,
-                 bp.bottom - bp.top, bp.right - bp.left};
-      es.push_back(Est<T>{(const T *)bursts[i].data, window(i), old, ws + est[i], keys + koff[i]});
-    }
-    if ((rc = launch_estimates<T>(s, es, 1))) return rc;
-    ChainArgs A{};
-    for (int i = 0; i < n; ++i) A.est[i] = est[i];
-    hipLaunchKernelGGL(k_ov_chain, dim3(1), dim3(64), 0, s, (const Workspace *)ws, A, n, offs);
-    S1_LAUNCH_CHECK("k_ov_chain");
-  }
   for (int i = 1; i < n; ++i) {
     const s1ifg_stitch_burst &bi = bursts[i];
     const T *bd = (const T *)bi.data;
     const Win w = window(i);
-    const float *offp = offs + i;
-    if (!chained) {
-      if (bi.estimate) {
-        Old<T> old{(const T *)mosaic, (long)mw, 0, 0, mh, mw};
-        std::vector<Est<T>> es{Est<T>{bd, w, old, ws, keys}};
-        if ((rc = launch_estimates<T>(s, es, 0))) return rc;
-      }
-      offp = &ws->st.off;
-    } else if (nest == 0) {
-      offp = &ws->st.off;  // 0: nothing estimated
+    // the boundaries one after the other against the mosaic
+    if (bi.estimate) {
+      Old<T> old{(const T *)mosaic, (long)mw, 0, 0, mh, mw};
+      std::vector<Est<T>> es{Est<T>{bd, w, old, ws, keys}};
+      if ((rc = launch_estimates<T>(s, es, 0))) return rc;
     }
+    const float *offp = &ws->st.off;
     const bool pairs = sizeof(T) == 8 && w.w % 2 == 0 && w.c0 % 2 == 0 && w.ld % 2 == 0 &&
                        (uintptr_t)bd % 16 == 0 && (uintptr_t)mosaic % 16 == 0;
     if (pairs) {
