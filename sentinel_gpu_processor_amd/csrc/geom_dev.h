@@ -373,18 +373,20 @@ __device__ __forceinline__ bool sample_valid(const GeomDev &g, double rg, double
 }
 
 // sin / cos of the flattening + reramp phase (geo2rdr.cu:210-219) at the
-// sampling position.
-__device__ __forceinline__ void reramp_phase(const GeomDev &g, double rng, double drg, double daz,
-                                             double t, float &sp, float &cp) {
-  double rp = rng + drg * g.dmrg;
-  const double trp = rp * (2.0 / kSol);  // two-way range time
-  double dt = trp - g.fm[0];
-  double ka = g.fm[1] + dt * (g.fm[2] + g.fm[3] * dt);
-  dt = trp - g.dc[0];
-  double fnc = g.dc[1] + dt * (g.dc[2] + g.dc[3] * dt);
-  double kt = ka * g.ks * rcp64(ka - g.ks);
-  double etac = -fnc * rcp64(ka);
-  double etaref = etac - g.etac0;
+// sampling position rg (r0 = floor(rg), a valid position: r0 + 1 < samples).
+// The reference evaluates kt and etaref at the range time of the shifted
+// range rp = rng + drg dmrg, i.e. slant_range_time + rg / fs: exactly the
+// range times of the burst's per-sample kt / etaref tables (deramp set-up,
+// geo2rdr.cu:533-542) at fractional index rg.  Both are smooth in range
+// (polynomials of degree 2 through a ratio), so their linear interpolation
+// between samples r0 and r0 + 1 is exact to ~1e-12 relative, far below the
+// FP64 phase resolution that matters (1e-8 rad), and replaces two
+// polynomials and two FP64 reciprocals per pixel.
+__device__ __forceinline__ void reramp_phase(const GeomDev &g, double rng, double rg, int r0,
+                                             double daz, double t, float &sp, float &cp) {
+  const double f = rg - r0;
+  const double k0 = g.kt[r0], k1 = g.kt[r0 + 1], e0 = g.etaref[r0], e1 = g.etaref[r0 + 1];
+  const double kt = fma(f, k1 - k0, k0), etaref = fma(f, e1 - e0, e0);
   double eta = -g.lines * g.dtaz / 2. + (t - g.tstart) + daz * g.dtaz;
   double etadiff = eta - etaref;
   double reramp = -kPi * kt * etadiff * etadiff;
@@ -403,7 +405,7 @@ __device__ __forceinline__ bool geocode_geom(const GeomDev &g, const OffPoly &op
   double t, drg, daz;
   geo_position(g, op, X, t, rng, rg, az, drg, daz);
   if (!sample_valid<INTERP>(g, rg, az)) return false;
-  reramp_phase(g, rng, drg, daz, t, sp, cp);
+  reramp_phase(g, rng, rg, (int)floor(rg), daz, t, sp, cp);
   return true;
 }
 

@@ -196,8 +196,8 @@ __device__ __forceinline__ void pair_tile_sinc(
         sample_position(G0, none, t, dr, rng, rg, az, drg, daz);
         float s, cc;
         if (!out0 && sample_valid<S1IFG_INTERP_SINC>(G0, rg, az)) {
-          reramp_phase(G0, rng, drg, daz, t, s, cc);
           const int r0 = (int)floor(rg), a0 = (int)floor(az);
+          reramp_phase(G0, rng, rg, r0, daz, t, s, cc);
           ir[p] = r0;
           ia[p] = a0;
           fr[p] = (float)(rg - r0);
@@ -274,8 +274,8 @@ __device__ __forceinline__ void pair_tile_sinc(
         sample_position(G1, op1, t, dr, rng, rg, az, drg, daz);
         float s, cc;
         if (sample_valid<S1IFG_INTERP_SINC>(G1, rg, az)) {
-          reramp_phase(G1, rng, drg, daz, t, s, cc);
           const int r0 = (int)floor(rg), a0 = (int)floor(az);
+          reramp_phase(G1, rng, rg, r0, daz, t, s, cc);
           ir[p] = r0;
           ia[p] = a0;
           fr[p] = (float)(rg - r0);
