@@ -55,6 +55,9 @@ def parse():
     p.add_argument("--ps-steps", type=int, default=10)
     p.add_argument("--stack-dates", type=int, default=20)
     p.add_argument("--gold-mode", default="auto", choices=["auto", "fp32", "fp16_split"])
+    p.add_argument("--batch-deramp", action="store_true",
+                   help="batched forms with a deramp pass (default: raw bursts, deramped as "
+                        "the pair kernel stages them)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", default="c2", choices=["c2", "c1"])
     p.add_argument("--calibrate-copy", action="store_true",
@@ -108,6 +111,12 @@ def b_alg(L, N, dem_px, rl, cl, dem_bytes=4):
     n = L * N
     nml = (L // rl) * (N // cl)
     return 16 * n + dem_bytes * dem_px + 20 * nml
+
+
+def batch_raw(args, rl, cl):
+    """Batched forms read the raw bursts (no deramp pass) unless asked
+    otherwise; that needs the sinc tile kernel (rl * cl <= 4)."""
+    return not args.batch_deramp and args.interp == "sinc" and rl * cl <= 4
 
 
 def run_gpu(args):
@@ -210,7 +219,7 @@ def run_gpu(args):
                                  sec_deramped=dsec, ifg=ifg, pow_ref=pr, pow_sec=ps, filt=filt,
                                  coh=coh)],
                       interp=args.interp, rl=rl, cl=cl, alpha=0.5, gold_mode=gold_mode,
-                      coh_win=5, coh_fill="clamp")
+                      coh_win=5, coh_fill="clamp", raw=batch_raw(args, rl, cl))
         tlog = (lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)) if args.tune_log else None
         shape, tms, _ = autotune.tune(b, reps=2, log=tlog)
         print(f"[bench] batched launch shape {shape} ({tms:.3f} ms while tuning)",
@@ -469,8 +478,9 @@ def run_stack(args, gold_mode):
           file=sys.stderr, flush=True)
     ctx = N.Context(0)
     dem_t = torch.from_numpy(st["dem"]["data"]).to(dev)
+    raw = batch_raw(args, 1, 4)
     run = Stack(ctx, st["dates"], st["grid"], dem_t, st["dem"], interp=args.interp, rl=1, cl=4,
-                gold_mode=gold_mode, coh_win=5)
+                gold_mode=gold_mode, coh_win=5, raw=raw)
     shape, _, _ = autotune.tune(run.batch, reps=1)
     prods = run.products()
     grid = st["grid"]
@@ -483,11 +493,17 @@ def run_stack(args, gold_mode):
     def fused(ev):
         ev.mark("loop")
         for k, (a, b) in enumerate(run.pairs):
-            K.deramp(ctx, run.geoms[a], run.raw[a], out=run.deramped[a])
-            K.deramp(ctx, run.geoms[b], run.raw[b], out=run.deramped[b])
             o = prods[k]
-            K.pair(ctx, run.geoms[a], run.geoms[b], grid, dem_t, st["dem"], run.deramped[a],
-                   run.deramped[b], args.interp, 1, 4, out=(o["ifg"], o["pow_ref"], o["pow_sec"]))
+            if raw:
+                K.pair(ctx, run.geoms[a], run.geoms[b], grid, dem_t, st["dem"], run.raw[a],
+                       run.raw[b], args.interp, 1, 4, out=(o["ifg"], o["pow_ref"], o["pow_sec"]),
+                       raw=True)
+            else:
+                K.deramp(ctx, run.geoms[a], run.raw[a], out=run.deramped[a])
+                K.deramp(ctx, run.geoms[b], run.raw[b], out=run.deramped[b])
+                K.pair(ctx, run.geoms[a], run.geoms[b], grid, dem_t, st["dem"], run.deramped[a],
+                       run.deramped[b], args.interp, 1, 4,
+                       out=(o["ifg"], o["pow_ref"], o["pow_sec"]))
             K.goldstein(ctx, o["ifg"], 0.5, mode=gold_mode, out=o["filt"])
             K.coherence(ctx, o["ifg"], o["pow_ref"], o["pow_sec"], win=5, out=o["coh"])
         ev.mark(None)
@@ -696,7 +712,7 @@ def run_iw(args, gold_mode):
         ev.mark(None)
 
     b = PairBatch(ctx, pairs, interp=args.interp, rl=rl, cl=cl, alpha=0.5, gold_mode=gold_mode,
-                  coh_win=5, coh_fill="clamp")
+                  coh_win=5, coh_fill="clamp", raw=batch_raw(args, rl, cl))
     shape, tms, _ = autotune.tune(b, reps=1)
     print(f"[bench] iw batched launch shape {shape} ({tms:.3f} ms while tuning)",
           file=sys.stderr, flush=True)

@@ -352,6 +352,11 @@ typedef struct {
   int32_t pair_blocks_per_cu;/* persistent grid of the pair kernel */
   int32_t gold_blocks_per_cu;/* persistent grid of the Goldstein patch pass */
   int32_t stream_blocks_per_cu; /* deramp / overlap-add / coherence */
+  int32_t raw;               /* != 0: no deramp pass; the pair kernel reads
+                              * ref_slc / sec_slc and deramps the samples as
+                              * it stages them (as s1ifg_pair_raw; sinc
+                              * interpolation with rl * cl <= 4 only), and
+                              * ref_deramped / sec_deramped may be NULL */
 } s1ifg_batch_opts;
 
 /* A *_blocks_per_cu value: one block per work item (a grid as large as the

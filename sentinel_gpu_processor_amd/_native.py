@@ -91,7 +91,7 @@ class BatchOpts(ctypes.Structure):
                 ("coh_win", ctypes.c_int32), ("coh_fill", ctypes.c_int32),
                 ("coh_seed", ctypes.c_uint32), ("pair_block_x", ctypes.c_int32),
                 ("pair_blocks_per_cu", ctypes.c_int32), ("gold_blocks_per_cu", ctypes.c_int32),
-                ("stream_blocks_per_cu", ctypes.c_int32)]
+                ("stream_blocks_per_cu", ctypes.c_int32), ("raw", ctypes.c_int32)]
 
 
 class StitchBurst(ctypes.Structure):

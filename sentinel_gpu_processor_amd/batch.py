@@ -3,7 +3,8 @@ through one launch sequence of persistent kernels, optionally replayed as
 one HIP graph (s1ifg_batch_* in include/s1ifg.h).
 
 Each pair's products are bit-identical to running kernels.deramp (x2),
-kernels.pair, kernels.goldstein and kernels.coherence on that pair alone; the
+kernels.pair, kernels.goldstein and kernels.coherence on that pair alone
+(with raw=True: kernels.pair(raw=True), no deramp pass); the
 batch exists to remove the per-burst launch and synchronisation overhead of
 the reference's loops (geo2rdr.cu:552-664, crossmul_daemon.cu:921-1117) and
 to fill the GPU with bursts of uneven size.
@@ -30,11 +31,13 @@ class PairBatch:
                         secondary's 6-coefficient offset polynomial
     Output buffers (ref_deramped, sec_deramped, ifg, pow_ref, pow_sec, filt,
     coh) are allocated unless given in the dict.
+    raw=True: the pair kernel deramps the raw bursts as it stages them (no
+    deramp pass, no deramped buffers; sinc interpolation, rl * cl <= 4).
     """
 
     def __init__(self, ctx, pairs, interp="sinc", rl=1, cl=4, alpha=0.5,
                  gold_mode="fp16_split", coh_win=5, coh_fill="hash", seed=0, launch=None,
-                 filt=True, coh=True):
+                 filt=True, coh=True, raw=False):
         self.ctx = ctx
         self.lib = ctx.lib
         self.handle = None
@@ -49,16 +52,18 @@ class PairBatch:
             q = dict(p)
             for side, b in (("ref", br), ("sec", bs)):
                 shape = (b["lines"], b["samples"])
-                raw = q.get(f"{side}_slc")
-                if q.get(f"{side}_deramped") is None:
-                    if raw is None:
+                src = q.get(f"{side}_slc")
+                if raw and src is None:
+                    raise ValueError(f"pair {k}: raw=True needs {side}_slc")
+                if q.get(f"{side}_deramped") is None and not raw:
+                    if src is None:
                         raise ValueError(f"pair {k}: {side}_slc or {side}_deramped is needed")
-                    q[f"{side}_deramped"] = torch.empty_like(raw)
+                    q[f"{side}_deramped"] = torch.empty_like(src)
                 for name in (f"{side}_slc", f"{side}_deramped"):
                     t = q.get(name)
                     if t is not None and (t.dtype != torch.complex64 or tuple(t.shape) != shape):
                         raise ValueError(f"pair {k}: {name} must be complex64 {shape}")
-            dev = q["ref_deramped"].device
+            dev = (q["ref_slc"] if raw else q["ref_deramped"]).device
             g = p["grid"]
             nr, nc = g["nrow"] // rl, g["ncol"] // cl
             if q.get("ifg") is None:
@@ -82,8 +87,8 @@ class PairBatch:
                 t.use_offpoly = 1
             t.ref_slc = _ptrval(q.get("ref_slc"))
             t.sec_slc = _ptrval(q.get("sec_slc"))
-            t.ref_deramped = _ptrval(q["ref_deramped"])
-            t.sec_deramped = _ptrval(q["sec_deramped"])
+            t.ref_deramped = _ptrval(q.get("ref_deramped"))
+            t.sec_deramped = _ptrval(q.get("sec_deramped"))
             t.ifg_ml = _ptrval(q["ifg"])
             t.pow_ref = _ptrval(q["pow_ref"])
             t.pow_sec = _ptrval(q["pow_sec"])
@@ -92,7 +97,7 @@ class PairBatch:
             self.pairs.append(q)
         o = N.BatchOpts(interp=N.INTERP[interp], rl=rl, cl=cl, alpha=alpha,
                         gold_mode=N.GOLD_MODE[gold_mode], coh_win=coh_win, coh_fill=fill,
-                        coh_seed=seed)
+                        coh_seed=seed, raw=int(bool(raw)))
         for key, v in (launch or {}).items():
             setattr(o, key, int(v))
         h = ctypes.c_void_p()
@@ -100,7 +105,8 @@ class PairBatch:
                                             ctypes.byref(h)))
         self.handle = h
         self.n = n
-        self.key = f"{interp}|{rl}x{cl}|{gold_mode}|coh{coh_win}|f{int(filt)}c{int(coh)}"
+        self.key = (f"{interp}|{rl}x{cl}|{gold_mode}|coh{coh_win}|f{int(filt)}c{int(coh)}"
+                    + ("|raw" if raw else ""))
 
     # ------------------------------------------------------------------
     def run(self, graph=True):

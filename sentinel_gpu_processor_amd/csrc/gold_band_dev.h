@@ -52,6 +52,22 @@ __host__ __device__ __forceinline__ long band_items(int pi_n, int pj_n) {
 __host__ __device__ __forceinline__ size_t band_partial_elems(int pi_n, int pj_n) {
   return (size_t)band_items(pi_n, pj_n) * kBandH * kBandCols;
 }
+// Per-image workspace: the partial buffer, then (256-byte aligned) the
+// work-queue counter of the persistent wavefronts.
+__host__ __device__ __forceinline__ size_t band_counter_offset(int pi_n, int pj_n) {
+  return (band_partial_elems(pi_n, pj_n) * 8 + 255) / 256 * 256;
+}
+
+// Next work item of a persistent wavefront: the first nw items go one to
+// each of the nw wavefronts; after that a wavefront takes the next item
+// from the queue counter (zeroed before the launch), so that wavefronts
+// whose items end early (patches of zeros) take more.
+template <typename C>
+__device__ __forceinline__ long band_next(C *ctr, long nw) {
+  unsigned k = 0;
+  if ((threadIdx.x & 63) == 0) k = (unsigned)atomicAdd(ctr, (C)1);
+  return nw + (long)(unsigned)__builtin_amdgcn_readfirstlane((int)k);
+}
 
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -85,6 +101,11 @@ __device__ __forceinline__ void band_item(const mf::Consts &K, const mf::SmoothL
     for (int p = p0; p < p1; ++p) {
       float2 x[16];
       mf::load_patch(img, nrow, ncol, pj_n, (long)p * pj_n + j, h, n, x);
+      // A patch of zeros (no data: the corners of a burst's latitude /
+      // longitude box, tiles outside the bursts) filters to zeros and adds
+      // nothing to the overlap-add: it is passed over (wavefront-uniform
+      // branch).
+      if (mf::patch_is_zero(x)) continue;
       float2 out[16];
       float norm;
       if (alpha == 0.5f)

@@ -258,6 +258,17 @@ __device__ __forceinline__ void load_patch(const float2 *__restrict__ img, int n
     for (int t = 0; t < 4; ++t) v[4 * q + t] = src[8 * q + t];
 }
 
+// Whether the patch held in x (lane layout) is all zeros, +0 or -0: the OR
+// of the component bit patterns without the sign bit is 0 in every lane (a
+// NaN or an infinity has exponent bits set).  Wavefront-uniform.
+__device__ __forceinline__ bool patch_is_zero(const float2 x[16]) {
+  unsigned b = 0;
+#pragma unroll
+  for (int r = 0; r < 16; ++r)
+    b |= __builtin_bit_cast(unsigned, x[r].x) | __builtin_bit_cast(unsigned, x[r].y);
+  return !__any((b & 0x7fffffffu) != 0);
+}
+
 // Filter one patch held in x (lane layout) into z' in the same layout:
 // lane (h, n) holds row n, columns 4h + 8q + t (q, t = 0..3) in out[4q + t];
 // z' norm is the filtered patch over n^2 (patch_weight adds the triangular

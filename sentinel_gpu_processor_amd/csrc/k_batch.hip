@@ -130,18 +130,21 @@ __global__ __launch_bounds__(256) void kb_pair(const S1C PairDev *__restrict__ P
 }
 
 // Goldstein FP16 split: (band, chunk) items of every pair on persistent
-// wavefronts, overlap-add fused (gold_band_dev.h).
+// wavefronts, overlap-add fused (gold_band_dev.h); after its first item a
+// wavefront takes the next from the stage's queue counter.
 __global__ __launch_bounds__(64 * kBandWaves) __attribute__((amdgpu_waves_per_eu(2))) void kb_gold_band(
-    const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, float alpha) {
+    const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, float alpha,
+    unsigned long long *ctr) {
   __shared__ BandLds L;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long total = off[n];
+  const long nw = (long)gridDim.x * kBandWaves;
   long t = (long)blockIdx.x * kBandWaves + wave;
   if (t >= total) return;
   mf::Consts K;
   mf::build_consts(lane >> 5, lane & 31, K, L.u[wave]);
   wave_sync_lds();  // L.u[wave] written
-  for (; t < total; t += (long)gridDim.x * kBandWaves) {
+  for (; t < total; t = band_next(ctr, nw)) {
     const int p = find_item(off, n, t);
     const PairDev &d = *(const PairDev *)&P[p];
     const long lt = t - off[p];
@@ -157,8 +160,9 @@ __global__ void kb_reset(unsigned long long *ctr, int n) {
 }
 
 // Sinc pair kernel with LDS-staged windows over 16 x 16-cell tiles of every
-// pair (rl * cl <= kPairMaxPix), one block per tile in tile order.
-template <int NP, int CL>
+// pair (rl * cl <= kPairMaxPix), one block per tile in tile order.  RAW:
+// ref_d / sec_d are the raw bursts, deramped as they are staged.
+template <int NP, int CL, bool RAW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_pair_tiles(
     const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, int rl, int cl) {
   __shared__ PairLds L;
@@ -169,7 +173,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const long lt = t - off[p];
   const int tiles_x = (d.nc_ml + kPairTile - 1) / kPairTile;
   const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
-  pair_tile_sinc<NP, CL>(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
+  pair_tile_sinc<NP, CL, RAW>(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
                  d.sec_d, rl, cl, ty, tx, d.nr_ml, d.nc_ml, d.ifg, d.pr, d.ps, L);
 }
 
@@ -178,7 +182,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // (cache reuse) whatever the grid size; the next tile is fetched while the
 // current one runs.  (A separate kernel: one inlined copy of the tile code
 // per kernel keeps either within the register budget.)
-template <int NP, int CL>
+template <int NP, int CL, bool RAW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_pair_tiles_q(
     const S1C PairDev *__restrict__ P, const S1C long *__restrict__ off, int n, int rl, int cl,
     unsigned long long *ctr) {
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const long lt = t - off[p];
     const int tiles_x = (d.nc_ml + kPairTile - 1) / kPairTile;
     const int ty = (int)(lt / tiles_x), tx = (int)(lt - (long)ty * tiles_x);
-    pair_tile_sinc<NP, CL>(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
+    pair_tile_sinc<NP, CL, RAW>(geo(d.gr), geo(d.gs), d.op, d.grid, d.dem, d.rows, d.cols, d.dh, d.ref_d,
                    d.sec_d, rl, cl, ty, tx, d.nr_ml, d.nc_ml, d.ifg, d.pr, d.ps, L);
     if (threadIdx.x == 0) slot[cur ^ 1] = nxt;
     cur ^= 1;
@@ -396,8 +400,10 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
     dim3 grd(grid_of(t, 1, ncu, o.pair_blocks_per_cu));
     // the 1 x 4 looks of the workloads with the look shape compiled in
     const bool l14 = o.rl == 1 && o.cl == 4;
-    auto tiles = l14 ? kb_pair_tiles<4, 4> : kb_pair_tiles<0, 0>;
-    auto tiles_q = l14 ? kb_pair_tiles_q<4, 4> : kb_pair_tiles_q<0, 0>;
+    auto tiles = o.raw ? (l14 ? kb_pair_tiles<4, 4, true> : kb_pair_tiles<0, 0, true>)
+                       : (l14 ? kb_pair_tiles<4, 4, false> : kb_pair_tiles<0, 0, false>);
+    auto tiles_q = o.raw ? (l14 ? kb_pair_tiles_q<4, 4, true> : kb_pair_tiles_q<0, 0, true>)
+                         : (l14 ? kb_pair_tiles_q<4, 4, false> : kb_pair_tiles_q<0, 0, false>);
     if (use_pair_tiles(o) && (long)grd.x >= t)
       hipLaunchKernelGGL(tiles, grd, dim3(256), 0, s, pairs, off[kStPair], n, o.rl, o.cl);
     else if (use_pair_tiles(o))
@@ -414,7 +420,8 @@ int enqueue(s1ifg_batch *b, hipStream_t s) {
   if (long t = total(kStGold)) {
     if (o.gold_mode == S1IFG_GOLD_FP16_SPLIT)
       hipLaunchKernelGGL(kb_gold_band, dim3(grid_of(t, kBandWaves, ncu, o.gold_blocks_per_cu)),
-                         dim3(64 * kBandWaves), 0, s, pairs, off[kStGold], n, o.alpha);
+                         dim3(64 * kBandWaves), 0, s, pairs, off[kStGold], n, o.alpha,
+                         b->d_ctr + kStGold);
     else
       hipLaunchKernelGGL(kb_gold_fp32, dim3(grid_of(t, kGoldGroups, ncu, o.gold_blocks_per_cu == S1IFG_BLOCK_PER_ITEM ? S1IFG_BLOCK_PER_ITEM : 8 * o.gold_blocks_per_cu)),
                          dim3(32 * kGoldGroups), 0, s, pairs, off[kStGold], n, o.alpha);
@@ -465,6 +472,10 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
     return set_error(S1IFG_ERANGE, "batched coherence window at most %d", 2 * kCohMaxHalo + 1);
   if (o.coh_fill < S1IFG_COH_FILL_CLAMP || o.coh_fill > S1IFG_COH_FILL_HASH)
     return set_error(S1IFG_EINVAL, "coherence fill %d", o.coh_fill);
+  o.raw = o.raw != 0;
+  if (o.raw && !use_pair_tiles(o))
+    return set_error(S1IFG_EINVAL, "raw bursts need sinc interpolation and rl * cl <= %d",
+                     kPairMaxPix);
 
   std::vector<PairDev> pairs(n);
   std::vector<ImgDev> imgs;
@@ -478,13 +489,15 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
     if ((rc = check_geom(ctx, t.g_ref)) || (rc = check_geom(ctx, t.g_sec))) return rc;
     if ((rc = make_grid(&t.grid, &d.grid)) || (rc = make_dem(&t.dem, &d.dem))) return rc;
     if ((rc = make_offpoly(t.use_offpoly ? t.offpoly_sec : nullptr, &d.op))) return rc;
-    if (!t.ref_deramped || !t.sec_deramped || !t.ifg_ml)
+    if (o.raw ? (!t.ref_slc || !t.sec_slc || !t.ifg_ml)
+              : (!t.ref_deramped || !t.sec_deramped || !t.ifg_ml))
       return set_error(S1IFG_EINVAL, "pair %d: null data pointer", p);
-    if (t.ref_deramped == t.ref_slc || t.sec_deramped == t.sec_slc ||
-        t.ref_deramped == t.sec_deramped)
+    if (!o.raw && (t.ref_deramped == t.ref_slc || t.sec_deramped == t.sec_slc ||
+                   t.ref_deramped == t.sec_deramped))
       return set_error(S1IFG_EINVAL, "pair %d: deramped bursts must not alias the raw bursts", p);
-    // distinct bursts to deramp (raw NULL: the deramped buffer is an input)
-    for (int side = 0; side < 2; ++side) {
+    // distinct bursts to deramp (raw NULL: the deramped buffer is an input);
+    // none with raw bursts
+    for (int side = 0; side < 2 && !o.raw; ++side) {
       const s1ifg_geom *g = side ? t.g_sec : t.g_ref;
       const void *raw = side ? (const void *)t.sec_slc : (const void *)t.ref_slc;
       void *der = side ? (void *)t.sec_deramped : (void *)t.ref_deramped;
@@ -510,8 +523,8 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
     d.gr = (const S1C GeomDev *)t.g_ref->d_dev;
     d.gs = (const S1C GeomDev *)t.g_sec->d_dev;
     d.dh = t.dh;
-    d.ref_d = (float2 *)t.ref_deramped;
-    d.sec_d = (float2 *)t.sec_deramped;
+    d.ref_d = (const float2 *)(o.raw ? t.ref_slc : t.ref_deramped);
+    d.sec_d = (const float2 *)(o.raw ? t.sec_slc : t.sec_deramped);
     d.ifg = (float2 *)t.ifg_ml;
     d.pr = t.pow_ref;
     d.ps = t.pow_sec;

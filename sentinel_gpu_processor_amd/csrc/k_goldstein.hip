@@ -59,9 +59,9 @@ int goldstein_grid(int nrow, int ncol, int *pi_n, int *pj_n) {
   return 0;
 }
 
-size_t band_partial_bytes(int pi_n, int pj_n);
+size_t band_workspace_bytes(int pi_n, int pj_n);
 int goldstein_band(s1ifg_ctx *ctx, const float2 *in, int nrow, int ncol, float alpha, int pi_n,
-                   int pj_n, float2 *partial, float2 *out);
+                   int pj_n, void *ws, float2 *out);
 
 }  // namespace s1ifg
 
@@ -83,8 +83,8 @@ extern "C" int s1ifg_goldstein(s1ifg_ctx *ctx, const s1ifg_cf32 *in, int nrow, i
   size_t npatch = (size_t)pi_n * pj_n;
   void *ws = nullptr;
   if (mode == S1IFG_GOLD_FP16_SPLIT) {
-    if ((rc = ctx_workspace(ctx, band_partial_bytes(pi_n, pj_n), &ws))) return rc;
-    return goldstein_band(ctx, (const float2 *)in, nrow, ncol, alpha, pi_n, pj_n, (float2 *)ws,
+    if ((rc = ctx_workspace(ctx, band_workspace_bytes(pi_n, pj_n), &ws))) return rc;
+    return goldstein_band(ctx, (const float2 *)in, nrow, ncol, alpha, pi_n, pj_n, ws,
                           (float2 *)out);
   }
   if ((rc = ctx_workspace(ctx, npatch * kGoldN * kGoldN * sizeof(float2), &ws))) return rc;

@@ -93,6 +93,50 @@ def test_batch_bit_identical_to_single_pairs(ctx, pairs, interp, gold_mode, win)
     b.close()
 
 
+def _single_raw(ctx, p, win):
+    from sentinel_gpu_processor_amd import kernels as K
+    ifg, pr, ps = K.pair(ctx, p["g_ref"], p["g_sec"], p["grid"], p["dem_t"], p["dem"],
+                         p["ref_slc"], p["sec_slc"], "sinc", 1, 4, offpoly=p["offpoly"],
+                         raw=True)
+    filt = K.goldstein(ctx, ifg, alpha=ALPHA, mode="fp16_split")
+    coh = K.coherence(ctx, ifg, pr, ps, win=win, fill="hash", seed=SEED)
+    return dict(ifg=ifg, pow_ref=pr, pow_sec=ps, filt=filt, coh=coh)
+
+
+def test_batch_raw_bit_identical_to_single_pairs(ctx, pairs):
+    """raw=True: no deramp pass, the pair kernel deramps the raw bursts as it
+    stages them; each pair equals kernels.pair(raw=True) + Goldstein +
+    coherence, eagerly, on graph replays and for two launch shapes (one
+    block per tile, persistent blocks on the tile queue)."""
+    from sentinel_gpu_processor_amd import _native as N
+    from sentinel_gpu_processor_amd.batch import PairBatch
+    want = [_single_raw(ctx, p, 5) for p in pairs]
+    b = PairBatch(ctx, pairs, interp="sinc", rl=1, cl=4, alpha=ALPHA, gold_mode="fp16_split",
+                  coh_win=5, coh_fill="hash", seed=SEED, raw=True)
+    assert all(o.get("ref_deramped") is None for o in b.pairs)
+    b.run(graph=False)
+    _assert_same(b, want, "raw eager")
+    for rep in range(3):
+        for o in b.outputs():
+            for t in o.values():
+                t.zero_()
+        b.run(graph=True)
+        torch.cuda.synchronize()
+        _ = torch.ones(1024, device="cuda").sum().item()
+        _assert_same(b, want, f"raw graph replay {rep}")
+    for shape in ({"pair_blocks_per_cu": 2, "gold_blocks_per_cu": 1},
+                  {"pair_blocks_per_cu": N.BLOCK_PER_ITEM, "gold_blocks_per_cu": N.BLOCK_PER_ITEM}):
+        b.launch = shape
+        b.run(graph=True)
+        _assert_same(b, want, f"raw shape {shape}")
+    for name, p in zip(NAMES, pairs):
+        assert np.array_equal(_bits(p["ref_slc"]), _bits(_dev(scene(name)["ref_slc"])))
+    b.close()
+    # raw bursts need the sinc tile kernel
+    with pytest.raises(N.S1IfgError, match="raw bursts"):
+        PairBatch(ctx, pairs[:1], interp="bilinear", raw=True)
+
+
 def test_batch_launch_shapes_do_not_change_results(ctx, pairs):
     from sentinel_gpu_processor_amd import _native as N
     from sentinel_gpu_processor_amd.batch import PairBatch

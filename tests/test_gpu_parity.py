@@ -248,6 +248,38 @@ def test_goldstein_scale_and_zero_patches(ctx, mode):
     assert np.all(got[82:, 104:] == 0)
 
 
+def test_goldstein_zero_patches_passed_over(ctx):
+    """The FP16-split pass skips patches whose components are all +0 or -0
+    (no-data regions, about half of a burst's look grid).  Against the FP32
+    path, which filters every patch: the same zero / non-finite pattern
+    (a NaN or an infinity inside an otherwise zero region still spreads over
+    its patches), values within the split tolerance elsewhere; and against
+    the oracle where the oracle is finite."""
+    from sentinel_gpu_processor_amd import kernels as K
+    rng = np.random.default_rng(11)
+    x = (rng.standard_normal((160, 224)) + 1j * rng.standard_normal((160, 224))).astype(np.complex64)
+    x[:72, :96] = 0                       # zero corner: every patch inside it is passed over
+    x[:72, :40] = -0.0 + 0j               # negative zeros are zeros too
+    x[100:, 150:] = 0
+    x[120, 200] = np.float32(np.nan)      # a NaN in a zero region
+    x[20, 20] = complex(np.inf, 0.0)      # an infinity in the zero corner
+    got = _np(K.goldstein(ctx, _dev(x), alpha=0.5, mode="fp16_split"))
+    f32 = _np(K.goldstein(ctx, _dev(x), alpha=0.5, mode="fp32"))
+    fin_g = np.isfinite(got.real) & np.isfinite(got.imag)
+    fin_f = np.isfinite(f32.real) & np.isfinite(f32.imag)
+    assert np.array_equal(fin_g, fin_f)
+    assert (~fin_g).any() and fin_g.any()
+    assert np.array_equal(got[fin_g] == 0, f32[fin_g] == 0)
+    assert (got[fin_g] == 0).any()
+    err = np.abs(got[fin_g] - f32[fin_g]).max() / np.abs(f32[fin_g]).max()
+    assert err < GOLD_TOL["fp16_split"][0] + GOLD_TOL["fp32"][0]
+    want, wsum = GS.goldstein(x, 32, 0.5)
+    m = (wsum > 0) & np.isfinite(want.real) & np.isfinite(want.imag)
+    assert np.array_equal(fin_g[wsum > 0], m[wsum > 0])
+    big = m & (np.abs(want) > 1e-3 * np.abs(want[m]).max())
+    assert phase_rms(got, want, big) < GOLD_TOL["fp16_split"][1]
+
+
 @pytest.mark.parametrize("mode", ["fp32", "fp16_split"])
 def test_goldstein_alpha0_identity(ctx, mode):
     from sentinel_gpu_processor_amd import kernels as K
