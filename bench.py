@@ -23,6 +23,7 @@ library launches on.
 """
 
 import argparse
+import gc
 import json
 import math
 import os
@@ -619,6 +620,10 @@ def cpu_baseline_stitch(bursts, mos):
 
 
 def time_form(fn, steps, warmup, name):
+    """W untimed steps, then K steps between device events.  Python's cyclic
+    garbage collector is run before the steps and held off during them: a
+    collection pass inside the timed region stalls the host between launches
+    (68 ms measured once) and leaves the device idle."""
     import torch
     for _ in range(warmup):
         fn(NoEvents())
@@ -626,14 +631,21 @@ def time_form(fn, steps, warmup, name):
     ev = StageEvents()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record()
-    for _ in range(steps):
-        fn(ev)
-    stop.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    gc.collect()
+    gc_was_on = gc.isenabled()
+    gc.disable()
+    try:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        start.record()
+        for _ in range(steps):
+            fn(ev)
+        stop.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    finally:
+        if gc_was_on:
+            gc.enable()
     ms = start.elapsed_time(stop) / steps
     r = dict(ms_per_step=ms, wall_s=wall, stages=ev.averages(steps))
     print(f"[bench] {name}: {ms:.3f} ms/step  stages(ms/step) "
@@ -875,6 +887,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(burst pairs shard across GPUs by process, see DESIGN.md)"}))
         sys.exit(2)
+    # The cyclic garbage collector runs only where asked (gc.collect before
+    # each timed loop): a pass that lands between two launches of a timed
+    # loop stalls the host for tens of ms and leaves the device idle.
+    gc.disable()
     res = run_gpu(args)
     L, Nr, dem_px, rl, cl = res["_shape"]
     npx = L * Nr

@@ -8,6 +8,7 @@ events on the context stream.  Results are cached in a JSON file keyed by
 the device, the processing options and the batch's size class.
 """
 
+import gc
 import json
 import os
 import time
@@ -54,16 +55,24 @@ def _save(path, table):
 
 
 def time_batch(batch, reps=2, graph=False):
-    """Mean device time (ms) of `reps` runs after one warm-up run."""
+    """Mean device time (ms) of `reps` runs after one warm-up run (Python's
+    garbage collector held off meanwhile: a collection pass between the
+    launches would leave the device idle and skew the comparison)."""
     stream = batch.ctx.stream
     batch.run(graph=graph)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        batch.run(graph=graph)
-    e1.record(stream)
-    e1.synchronize()
+    gc_was_on = gc.isenabled()
+    gc.disable()
+    try:
+        e0.record(stream)
+        for _ in range(reps):
+            batch.run(graph=graph)
+        e1.record(stream)
+        e1.synchronize()
+    finally:
+        if gc_was_on:
+            gc.enable()
     return e0.elapsed_time(e1) / reps
 
 

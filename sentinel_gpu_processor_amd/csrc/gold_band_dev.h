@@ -18,7 +18,10 @@
 // (8 kBandCW)) = 3.4 x the output in traffic (a patch buffer: 16 x; one
 // patch row per item: 4.75 x).  Larger groups and chunks cut that further
 // but leave too few items (C2: 93 x 98 = 9,114 for 2,048 wavefront slots)
-// to balance the chip.
+// to balance the chip.  Patches of zeros (no-data regions) are passed over;
+// an item writes its partial columns only from its first non-zero patch on
+// and records where (its zero mark), so no-data regions cost neither the
+// filter nor partial traffic.
 #pragma once
 
 #include "gold_dev.h"
@@ -52,10 +55,26 @@ __host__ __device__ __forceinline__ long band_items(int pi_n, int pj_n) {
 __host__ __device__ __forceinline__ size_t band_partial_elems(int pi_n, int pj_n) {
   return (size_t)band_items(pi_n, pj_n) * kBandH * kBandCols;
 }
-// Per-image workspace: the partial buffer, then (256-byte aligned) the
+// A pair's Goldstein buffer: the partial sums, then (256-byte aligned) one
+// int per item, its zero mark: the item's partial columns below it (from
+// the item's first column) are zeros and were not written (kBandCols: the
+// whole item is zeros).
+__host__ __device__ __forceinline__ size_t band_zb_offset(int pi_n, int pj_n) {
+  return (band_partial_elems(pi_n, pj_n) * 8 + 255) / 256 * 256;
+}
+__host__ __device__ __forceinline__ size_t band_buffer_bytes(int pi_n, int pj_n) {
+  return band_zb_offset(pi_n, pj_n) + ((size_t)band_items(pi_n, pj_n) * 4 + 255) / 256 * 256;
+}
+__host__ __device__ __forceinline__ int *band_zb(float2 *partial, int pi_n, int pj_n) {
+  return (int *)((char *)partial + band_zb_offset(pi_n, pj_n));
+}
+__host__ __device__ __forceinline__ const int *band_zb(const float2 *partial, int pi_n, int pj_n) {
+  return (const int *)((const char *)partial + band_zb_offset(pi_n, pj_n));
+}
+// Per-image workspace: the buffer above, then (256-byte aligned) the
 // work-queue counter of the persistent wavefronts.
 __host__ __device__ __forceinline__ size_t band_counter_offset(int pi_n, int pj_n) {
-  return (band_partial_elems(pi_n, pj_n) * 8 + 255) / 256 * 256;
+  return band_buffer_bytes(pi_n, pj_n);
 }
 
 // Next work item of a persistent wavefront: the first nw items go one to
@@ -76,8 +95,11 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 // One (group, chunk) item on one wavefront (no block barriers).  buf: the
-// wave's LDS buffer (kBandH x kBandLd); partial: this image's partial buffer
-// [ngroup][nchunk][kBandH][kBandCols].
+// wave's LDS buffer (kBandH x kBandLd); partial: this image's buffer,
+// partial sums [ngroup][nchunk][kBandH][kBandCols] then the items' zero
+// marks.  Until the item's first non-zero patch the overlap-add holds zeros:
+// the columns that become final before it are not written, and the zero
+// mark records where the written columns start.
 __device__ __forceinline__ void band_item(const mf::Consts &K, const mf::SmoothLds &U,
                                           const float2 *__restrict__ img, int nrow, int ncol,
                                           int pi_n, int pj_n, int g, int c, float alpha,
@@ -96,6 +118,7 @@ __device__ __forceinline__ void band_item(const mf::Consts &K, const mf::SmoothL
   for (int i = lane; i < kBandH * kBandLd; i += 64) buf[i] = make_float2(0.f, 0.f);
   wave_sync_lds();
   int lo = ox0;
+  int zb = -1;  // zero mark, set at the first non-zero patch (wavefront-uniform)
   for (int j = j0; j < j1; ++j) {
     const int ox = gold_origin(j, ncol);
     for (int p = p0; p < p1; ++p) {
@@ -106,6 +129,7 @@ __device__ __forceinline__ void band_item(const mf::Consts &K, const mf::SmoothL
       // nothing to the overlap-add: it is passed over (wavefront-uniform
       // branch).
       if (mf::patch_is_zero(x)) continue;
+      if (zb < 0) zb = lo - ox0;  // the columns flushed so far hold zeros
       float2 out[16];
       float norm;
       if (alpha == 0.5f)
@@ -128,24 +152,28 @@ __device__ __forceinline__ void band_item(const mf::Consts &K, const mf::SmoothL
         }
       wave_sync_lds();  // the next patch's lanes add to these entries
     }
-    // columns no later patch column of the chunk reaches are final
+    // columns no later patch column of the chunk reaches are final (while
+    // every patch so far was zeros they are zeros, not written)
     const int done = (j + 1 < j1) ? gold_origin(j + 1, ncol) : ox + kGoldN;
     const int w = done - lo;
-    for (int i = lane; i < w * hrows; i += 64) {
-      const int r = i / w, col = lo + (i - r * w);
-      float2 *e = buf + r * kBandLd + (col & (kBandBuf - 1));
-      dst[(size_t)r * kBandCols + (col - ox0)] = *e;
-      *e = make_float2(0.f, 0.f);
+    if (zb >= 0) {
+      for (int i = lane; i < w * hrows; i += 64) {
+        const int r = i / w, col = lo + (i - r * w);
+        float2 *e = buf + r * kBandLd + (col & (kBandBuf - 1));
+        dst[(size_t)r * kBandCols + (col - ox0)] = *e;
+        *e = make_float2(0.f, 0.f);
+      }
     }
     lo = done;
     wave_sync_lds();
   }
+  if (lane == 0) band_zb(partial, pi_n, pj_n)[(size_t)g * nchunk + c] = zb < 0 ? kBandCols : zb;
 }
 
 // Output pixels of a tile of kSumTX x kSumTY (block kSumTX x kSumTY): the
-// partial sums of the groups and chunks covering each pixel, in increasing
-// group then chunk order, over the triangular weight sum; 0 where the
-// weight sum is 0.  The factors that depend only on the column (the column
+// partial sums of the groups and chunks covering each pixel (zeros below an
+// item's zero mark, not read), in increasing group then chunk order, over
+// the triangular weight sum; 0 where the weight sum is 0.  The factors that depend only on the column (the column
 // weight terms and the <= 2 chunks covering it) or only on the row (the row
 // weight terms and the <= kSumGroups groups covering it) are computed once per tile
 // column / row into LDS.  Contains block barriers (the last one lets the
@@ -162,7 +190,12 @@ struct BandSumLds {
   int nch[kSumTX], ch[kSumTX][2], xo[kSumTX][2];
   float cy[kSumTY], sy[kSumTY];
   int nb[kSumTY], grp[kSumTY][kSumGroups], ty[kSumTY][kSumGroups];
+  // zero marks of the items the tile meets: groups from the first row's
+  // first, chunks from the first column's first (kSumTY rows advance the
+  // groups by at most one; kSumTX columns meet at most 3 chunks)
+  int zb[kSumGroups + 1][3];
 };
+static_assert(kSumTY <= kGoldN / 4 && kSumTX <= 8 * kBandCW, "zero-mark table bounds");
 
 __device__ __forceinline__ void band_sum_tile(const float2 *__restrict__ partial, int nrow,
                                               int ncol, int pi_n, int pj_n, int tile_x,
@@ -229,6 +262,19 @@ __device__ __forceinline__ void band_sum_tile(const float2 *__restrict__ partial
     L.nb[tx] = n;
   }
   __syncthreads();
+  // the zero marks of the items the tile meets (the tile's first row and
+  // column lie in the image, so both have covering patches)
+  const int g0 = L.grp[0][0], c0 = L.ch[0][0];
+  {
+    const int t = tyi * kSumTX + tx;
+    if (t < (kSumGroups + 1) * 3) {
+      const int g = g0 + t / 3, c = c0 + t % 3;
+      L.zb[t / 3][t % 3] = g < band_ngroup(pi_n) && c < nchunk
+                               ? band_zb(partial, pi_n, pj_n)[(size_t)g * nchunk + c]
+                               : kBandCols;
+    }
+  }
+  __syncthreads();
   if (x < ncol && y < nrow) {
     const float ws = L.cx[tx] * L.sy[tyi] + L.cy[tyi] * L.sx[tx];
     float2 r = make_float2(0.f, 0.f);
@@ -242,9 +288,14 @@ __device__ __forceinline__ void band_sum_tile(const float2 *__restrict__ partial
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
           v[i][k] = make_float2(0.f, 0.f);
-          if (i < nb && k < nch)
-            v[i][k] = partial[(((size_t)L.grp[tyi][i] * nchunk + L.ch[tx][k]) * kBandH +
-                               L.ty[tyi][i]) * kBandCols + L.xo[tx][k]];
+          if (i < nb && k < nch) {
+            // a column below the item's zero mark holds zeros, not written
+            const int gi = min(max(L.grp[tyi][i] - g0, 0), kSumGroups);
+            const int ci = min(max(L.ch[tx][k] - c0, 0), 2);
+            if (L.xo[tx][k] >= L.zb[gi][ci])
+              v[i][k] = partial[(((size_t)L.grp[tyi][i] * nchunk + L.ch[tx][k]) * kBandH +
+                                 L.ty[tyi][i]) * kBandCols + L.xo[tx][k]];
+          }
         }
       float ax = 0.f, ay = 0.f;
 #pragma unroll

@@ -544,7 +544,7 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
       goldstein_grid(d.nr_ml, d.nc_ml, &d.pi_n, &d.pj_n);
       patch_off[p] = patch_bytes;
       patch_bytes += align256(o.gold_mode == S1IFG_GOLD_FP16_SPLIT
-                                  ? band_partial_elems(d.pi_n, d.pj_n) * sizeof(float2)
+                                  ? band_buffer_bytes(d.pi_n, d.pj_n)
                                   : (size_t)d.pi_n * d.pj_n * kGoldN * kGoldN * sizeof(float2));
     }
     tab_off[p] = tab_bytes;
