@@ -284,6 +284,47 @@ __device__ __forceinline__ float2 deramp_rot(double eta, double kth, double etar
   return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
 }
 
+// exp(2 pi i rv) of a phase rv in revolutions (FP64), reduced to [-1/2, 1/2]
+// before the hardware cosine and sine.
+__device__ __forceinline__ float2 phasor_rev(double rv) {
+  const float r = (float)(rv - rint(rv));
+  return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
+}
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// The deramp phasors of one column (kth = -kt / 2, etaref) at the azimuth
+// times eta0 + k h, k = 0, 1, ...: z_k = exp(2 pi i kth (eta0 + k h -
+// etaref)^2) by a second-order recurrence, z_{k+1} = z_k u_k, u_{k+1} = u_k q,
+// with z_0, u_0 = exp(2 pi i kth h (2 d + h)) (d = eta0 - etaref) and q =
+// exp(2 pi i 2 kth h^2) from FP64 phases.  Each step rounds two complex
+// products in FP32: after k steps the phasor is within ~k 1e-7 of
+// deramp_rot's (a few 1e-6 over a staged column), for 2 complex products
+// per sample instead of an FP64 phase and a sine and cosine.
+struct DerampSeq {
+  float2 z, u, q;
+};
+
+__device__ __forceinline__ DerampSeq deramp_seq(double eta0, double h, double kth,
+                                                double etaref) {
+  const double d = eta0 - etaref;
+  DerampSeq s;
+  s.z = phasor_rev(kth * d * d);
+  s.u = phasor_rev(kth * h * (2.0 * d + h));
+  s.q = phasor_rev(2.0 * kth * h * h);
+  return s;
+}
+
+// The next sample of the column deramped: a z_k, then the step to k + 1.
+__device__ __forceinline__ float2 deramp_next(DerampSeq &s, float2 a) {
+  const float2 o = cmulf(a, s.z);
+  s.z = cmulf(s.z, s.u);
+  s.u = cmulf(s.u, s.q);
+  return o;
+}
+
 // Sample (row, col) of a raw burst, deramped (deramp_rot).
 __device__ __forceinline__ float2 raw_tap(const GeomDev &g, const float2 *__restrict__ z, int row,
                                           int col) {
@@ -437,35 +478,35 @@ __device__ __forceinline__ float2 sinc_interp(const float2 *__restrict__ base, l
   return make_float2(rx * c - ry * s, rx * s + ry * c);
 }
 
-// sinc_interp on a raw burst: each of the 8 x 8 taps deramped as it is read
-// (the columns' kt and etaref read once).
+// sinc_interp on a raw burst: the taps deramped as they are read, column
+// by column (a deramp_seq down each of the 8 tap columns), so the azimuth
+// sums come first, then the range sum (the same separable weights; the
+// summation order differs from sinc_interp's, so the result agrees with it
+// to FP32 rounding, not bit for bit).
 __device__ __forceinline__ float2 sinc_interp_raw(const GeomDev &g, const float2 *__restrict__ z,
                                                   int ia, int ir, float fr, float fa, float sp,
                                                   float cp) {
   float wr[8], wa[8];
   const float nr = sinc_weights(fr, wr);
   const float na = sinc_weights(fa, wa);
-  double kth[8], er[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    kth[k] = -0.5 * g.kt[ir - 3 + k];
-    er[k] = g.etaref[ir - 3 + k];
-  }
+  const double eta0 = -g.lines * g.ati / 2. + (ia - 3) * g.ati;
+  const float2 *base = z + (size_t)(ia - 3) * g.samples + (ir - 3);
   float rx = 0.f, ry = 0.f;
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const int row = ia - 3 + a;
-    const double eta = -g.lines * g.ati / 2. + row * g.ati;
-    const float2 *rp = z + (size_t)row * g.samples + (ir - 3);
+  for (int k = 0; k < 8; ++k) {
+    DerampSeq q = deramp_seq(eta0, g.ati, -0.5 * g.kt[ir - 3 + k], g.etaref[ir - 3 + k]);
+    float2 t[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) t[a] = base[(size_t)a * g.samples + k];
     float ax = 0.f, ay = 0.f;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const float2 v = deramp_rot(eta, kth[k], er[k], rp[k]);
-      ax = fmaf(wr[k], v.x, ax);
-      ay = fmaf(wr[k], v.y, ay);
+    for (int a = 0; a < 8; ++a) {
+      const float2 v = deramp_next(q, t[a]);
+      ax = fmaf(wa[a], v.x, ax);
+      ay = fmaf(wa[a], v.y, ay);
     }
-    rx = fmaf(wa[a], ax, rx);
-    ry = fmaf(wa[a], ay, ry);
+    rx = fmaf(wr[k], ax, rx);
+    ry = fmaf(wr[k], ay, ry);
   }
   const float sc = nr * na, c = cp * sc, s = sp * sc;
   return make_float2(rx * c - ry * s, rx * s + ry * c);

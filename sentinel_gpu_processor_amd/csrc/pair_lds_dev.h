@@ -64,10 +64,10 @@ __device__ __forceinline__ void stage_window(const float2 *__restrict__ z, int n
   }
 }
 
-// stage_window from a raw burst, deramped as it is staged (deramp_rot):
-// wavefront w stages rows w, w + 4, ...; its lanes take consecutive columns
-// (coalesced rows), so a lane keeps its column's kt / etaref across the rows
-// it stages, and kStage rows' loads are issued before their rotations.
+// stage_window from a raw burst, deramped as it is staged: wavefront w
+// stages rows w, w + 4, ...; its lanes take consecutive columns (coalesced
+// rows), a lane's rotations follow its column's deramp_seq with a step of
+// 4 rows, and kStage rows' loads are issued before their rotations.
 __device__ __forceinline__ void stage_window_deramp(const float2 *__restrict__ z,
                                                     const GeomDev &g, int wa0, int wr0, int W,
                                                     int H, float2 *win) {
@@ -76,7 +76,8 @@ __device__ __forceinline__ void stage_window_deramp(const float2 *__restrict__ z
   const double eta0 = -g.lines * g.ati / 2.;
   for (int c = lane; c < W; c += 64) {
     const int col = wr0 + c;
-    const double kth = -0.5 * g.kt[col], er = g.etaref[col];
+    DerampSeq q =
+        deramp_seq(eta0 + (wa0 + wave) * g.ati, 4 * g.ati, -0.5 * g.kt[col], g.etaref[col]);
     for (int r0 = wave; r0 < H; r0 += 4 * kStage) {
       float2 v[kStage];
 #pragma unroll
@@ -87,7 +88,7 @@ __device__ __forceinline__ void stage_window_deramp(const float2 *__restrict__ z
 #pragma unroll
       for (int k = 0; k < kStage; ++k) {
         const int r = r0 + 4 * k;
-        if (r < H) win[r * W + c] = deramp_rot(eta0 + (wa0 + r) * g.ati, kth, er, v[k]);
+        if (r < H) win[r * W + c] = deramp_next(q, v[k]);
       }
     }
   }
