@@ -246,9 +246,10 @@ int s1ifg_coherence(s1ifg_ctx *ctx, const s1ifg_cf32 *ifg_ml,
 /* A18 Goldstein adaptive filter (patch n_win with step n_win/4, alpha), with
  * a deterministic gather-form overlap-add.  mode: S1IFG_GOLD_FP32 (radix-2
  * FFT in FP32) or S1IFG_GOLD_FP16_SPLIT (DFT-by-GEMM on MFMA, hi/lo split
- * FP16 operands, FP32 accumulation).  Pixels whose weight sum is 0 are 0.
- * n_win must be 32; nrow, ncol >= n_win.  Replaces goldstein_filter_cuda_host
- * and its kernels (goldstein.cu:87-408). */
+ * FP16 operands, FP32 accumulation; patches whose components are all +0 or
+ * -0 filter to zeros and are passed over).  Pixels whose weight sum is 0
+ * are 0.  n_win must be 32; nrow, ncol >= n_win.  Replaces
+ * goldstein_filter_cuda_host and its kernels (goldstein.cu:87-408). */
 int s1ifg_goldstein(s1ifg_ctx *ctx, const s1ifg_cf32 *in, int nrow, int ncol,
                     int n_win, float alpha, int mode, s1ifg_cf32 *out);
 
