@@ -104,6 +104,21 @@ def profiled(kind):
     return {}
 
 
+def nonzero_patches(torch, img, n=32, step=8):
+    """(patches with a non-zero component, all patches) of the Goldstein
+    grid of img (origins min(j step, size - n), as the filter's)."""
+    F = torch.nn.functional
+    a = torch.view_as_real(img).abs().amax(-1)
+    m = F.max_pool2d(F.max_pool2d(a[None, None], (n, 1), 1), (1, n), 1)[0, 0]
+    nr, nc = img.shape
+    oy = torch.tensor([min(i * step, nr - n) for i in range(-(-(nr - n) // step) + 1)],
+                      device=img.device)
+    ox = torch.tensor([min(j * step, nc - n) for j in range(-(-(nc - n) // step) + 1)],
+                      device=img.device)
+    pm = m[oy][:, ox]
+    return int((pm > 0).sum().item()), int(pm.numel())
+
+
 def gold_patches(nrow, ncol, n=32, step=8):
     """Patch count of the Goldstein grid (origins min(j step, size - n))."""
     return (-(-(nrow - n) // step) + 1) * (-(-(ncol - n) // step) + 1)
@@ -232,6 +247,10 @@ def run_gpu(args):
         results[name] = time_form(forms[name], args.steps, args.warmup, name)
     results["_gold_mode"] = gold_mode
     results["_shape"] = (L, Nr, dem["data"].size, rl, cl)
+    # Goldstein patches the MFMA pass filters: the ones with a non-zero
+    # component (patches of zeros, the no-data corners of the look grid,
+    # are passed over)
+    results["_gold_patches"] = nonzero_patches(torch, ifg)
     if "batch" in holder:
         holder["batch"].close()
     if not args.no_ingest:
@@ -935,9 +954,24 @@ def main():
             kernels[st].update(fp64_tflops=round(fp64[st] / t_launch / 1e12, 2),
                                fp64_frac=round(fp64[st] / t_launch / FP64_PEAK, 4))
         if st == "goldstein" and res["_gold_mode"] == "fp16_split":
-            fl = gold_patches(L // rl, Nr // cl) * GOLD_MFMA_FLOPS_PER_PATCH
+            # MFMA work of the filtered (non-zero) patches only
+            nz, ntot = res["_gold_patches"]
+            fl = nz * GOLD_MFMA_FLOPS_PER_PATCH
             kernels[st].update(mfma_tflops=round(fl / t_launch / 1e12, 1),
-                               mfma_frac=round(fl / t_launch / FP16_DENSE_PEAK, 4))
+                               mfma_frac=round(fl / t_launch / FP16_DENSE_PEAK, 4),
+                               patches_filtered=nz, patches_total=ntot)
+    if rform != "fused" and "fused" in res and "pair" in res["fused"]["stages"]:
+        # the pair kernel on deramped bursts too (without the deramp it absorbs
+        # in the raw form)
+        t_launch = res["fused"]["stages"]["pair"] * 1e-3
+        ach = sb["pair"] / t_launch / 1e9
+        kernels["pair"] = dict(ms_per_launch=round(t_launch * 1e3, 4), alg_bytes_per_launch=sb["pair"],
+                               achieved_gbs=round(ach, 1), hbm_frac=round(ach * 1e9 / HBM_PEAK, 4),
+                               traffic=traffic.get("pair"), valu_issue_busy=valu.get("pair"),
+                               form="fused")
+        if "pair" in fp64:
+            kernels["pair"].update(fp64_tflops=round(fp64["pair"] / t_launch / 1e12, 2),
+                                   fp64_frac=round(fp64["pair"] / t_launch / FP64_PEAK, 4))
     k = kernels[dom]
     roof = dict(kernel=dom, form=rform, bound="hbm", achieved=k["achieved_gbs"], peak=HBM_PEAK / 1e9,
                 unit="GB/s", frac=k["hbm_frac"], traffic=k["traffic"],
