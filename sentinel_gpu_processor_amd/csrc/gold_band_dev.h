@@ -137,19 +137,26 @@ __device__ __forceinline__ void band_item(const mf::Consts &K, const mf::SmoothL
       else
         mf::filter_patch_regs<false>(K, U, x, h, n, alpha, out, norm);
       // add the weighted patch (within a patch each lane owns distinct
-      // entries)
+      // entries): per group of 4 columns, the 4 reads are issued before
+      // the updates, so the read-modify-write round trips to LDS overlap
       float2 *row = buf + (gold_origin(p, nrow) - oy0 + n) * kBandLd;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 4; ++q) {
+        float2 *e[4];
+        float2 v[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          float2 *e = row + ((ox + 4 * h + 8 * q + t) & (kBandBuf - 1));
-          const float w = mf::patch_weight(h, n, q, t, norm);
-          float2 v = *e;
-          v.x = fmaf(out[4 * q + t].x, w, v.x);
-          v.y = fmaf(out[4 * q + t].y, w, v.y);
-          *e = v;
+          e[t] = row + ((ox + 4 * h + 8 * q + t) & (kBandBuf - 1));
+          v[t] = *e[t];
         }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const float w = mf::patch_weight(h, n, q, t, norm);
+          v[t].x = fmaf(out[4 * q + t].x, w, v[t].x);
+          v[t].y = fmaf(out[4 * q + t].y, w, v[t].y);
+          *e[t] = v[t];
+        }
+      }
       wave_sync_lds();  // the next patch's lanes add to these entries
     }
     // columns no later patch column of the chunk reaches are final (while
