@@ -743,8 +743,19 @@ def run_iw(args, gold_mode):
             K.coherence(ctx, p["ifg"], p["pow_ref"], p["pow_sec"], win=5, out=p["coh"])
         ev.mark(None)
 
+    def fused_raw(ev):
+        ev.mark("fused_raw")
+        for p in pairs:
+            K.pair(ctx, p["g_ref"], p["g_sec"], p["grid"], p["dem_t"], p["dem"],
+                   p["ref_slc"], p["sec_slc"], args.interp, rl, cl, offpoly=p["offpoly"],
+                   out=(p["ifg"], p["pow_ref"], p["pow_sec"]), raw=True)
+            K.goldstein(ctx, p["ifg"], 0.5, mode=gold_mode, out=p["filt"])
+            K.coherence(ctx, p["ifg"], p["pow_ref"], p["pow_sec"], win=5, out=p["coh"])
+        ev.mark(None)
+
+    raw = batch_raw(args, rl, cl)
     b = PairBatch(ctx, pairs, interp=args.interp, rl=rl, cl=cl, alpha=0.5, gold_mode=gold_mode,
-                  coh_win=5, coh_fill="clamp", raw=batch_raw(args, rl, cl))
+                  coh_win=5, coh_fill="clamp", raw=raw)
     shape, tms, _ = autotune.tune(b, reps=1)
     print(f"[bench] iw batched launch shape {shape} ({tms:.3f} ms while tuning)",
           file=sys.stderr, flush=True)
@@ -755,7 +766,8 @@ def run_iw(args, gold_mode):
         ev.mark(None)
 
     out = {}
-    for name, fn in (("plain", plain), ("fused", fused), ("batched", batched)):
+    loops = [("plain", plain), ("fused", fused)] + ([("fused_raw", fused_raw)] if raw else [])
+    for name, fn in loops + [("batched", batched)]:
         r = time_form(fn, args.iw_steps, 1, "iw " + name)
         out[name] = dict(ms_per_step=round(r["ms_per_step"], 3),
                          mpix_s=round(npx / (r["ms_per_step"] * 1e-3) / 1e6, 1))
@@ -771,8 +783,10 @@ def run_iw(args, gold_mode):
                         f"{max(p['grid']['ncol'] for p in pairs)}, {args.interp}, {gold_mode}",
                target_px=npx, b_alg_bytes=balg, launch=shape, forms=out,
                batched_vs_plain=round(out["plain"]["ms_per_step"] / out["batched"]["ms_per_step"], 3),
-               batched_vs_fused_loop=round(out["fused"]["ms_per_step"]
+               batched_vs_fused_loop=round(min(v["ms_per_step"] for k, v in out.items()
+                                               if k.startswith("fused"))
                                            / out["batched"]["ms_per_step"], 3),
+               batched_raw=raw,
                scratch_bytes=b.scratch_bytes())
     b.close()
     return res
