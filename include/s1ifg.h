@@ -128,9 +128,11 @@ int s1ifg_ctx_sync(s1ifg_ctx *ctx);
  * 16 x 16-cell tiles: [0] tiles, [1] burst-tiles skipped by the tile's
  * corner test, [2] burst-tiles interpolated from the LDS-staged tap window,
  * [3] burst-tiles whose window exceeds the LDS budget (taps read from
- * global memory), [4] burst-tiles not skipped without any valid pixel.
- * NULL turns counting off (the default).  The caller zeroes the array. */
-#define S1IFG_TILE_STATS 5
+ * global memory), [4] burst-tiles not skipped without any valid pixel, [5]
+ * burst-tiles of s1ifg_pair_raw interpolated from a sheared LDS window (the
+ * rectangular one exceeds the budget, the sheared one fits).  NULL turns
+ * counting off (the default).  The caller zeroes the array. */
+#define S1IFG_TILE_STATS 6
 int s1ifg_ctx_set_tile_stats(s1ifg_ctx *ctx, uint64_t *counts);
 /* Workspace currently held by the context, in bytes. */
 int s1ifg_ctx_workspace(s1ifg_ctx *ctx, int64_t *bytes);

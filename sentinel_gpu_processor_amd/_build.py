@@ -2,6 +2,14 @@
 
 Every HIP translation unit is compiled for gfx950 with packed-FP32
 instructions disabled, then linked into ``sentinel_gpu_processor_amd/lib``.
+
+Floating-point contraction is expression-local (-ffp-contract=on): a * b + c
+inside one expression becomes one FMA, and nothing is fused across
+statements.  HIP's default (fuse wherever the optimiser sees a product
+feeding a sum) decides per kernel, after inlining, so one device function
+could round differently in the per-pair and the batched kernels; with
+expression-local contraction every kernel that inlines it computes the same
+bits.
 """
 
 import concurrent.futures as cf
@@ -22,6 +30,7 @@ SOURCES = ["host.cpp", "k_geom.hip", "k_ifg.hip", "k_goldstein.hip", "k_gold_mfm
 HIPFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-ffp-contract=on",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 

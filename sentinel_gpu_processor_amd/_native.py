@@ -21,7 +21,7 @@ GOLD_FP32 = 0
 GOLD_FP16_SPLIT = 1
 DEM_F32 = 0
 DEM_I16 = 1
-TILE_STATS = 5  # counters of s1ifg_ctx_set_tile_stats
+TILE_STATS = 6  # counters of s1ifg_ctx_set_tile_stats
 COH_FILL_CLAMP = 0
 COH_FILL_NAN = 1
 COH_FILL_HASH = 2

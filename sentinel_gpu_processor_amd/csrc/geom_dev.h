@@ -590,8 +590,15 @@ __device__ __forceinline__ void pixel_xyz(const GridDev &gr, const DemDev &dem,
 // all 64 lanes of a wavefront; the result is uniform.  A DEM box or a tile
 // of more than kSkipDemMax samples, or a corner whose Newton iteration does
 // not converge, gives false.
+//
+// skew (optional, uniform): the burst's range samples per azimuth line
+// along the tile's first grid column (corners (r0, c0) and (r1, c0) at the
+// lowest height), rounded to an integer in [-kSkewMax, kSkewMax]; 0 when
+// the corners were not solved or span less than one line.  The pair kernel
+// shears a tap window that exceeds its LDS budget by it.
 constexpr double kSkipMargin = 16.0;
 constexpr int kSkipDemMax = 4096;
+constexpr int kSkewMax = 8;
 
 __device__ __forceinline__ int dem_index(double f, int n) {
   f = fmin(fmax(f, 0.0), (double)(n - 1));
@@ -602,8 +609,9 @@ template <int INTERP>
 __device__ __forceinline__ bool tile_outside(const GeomDev &g, const OffPoly &op, const DemDev &dem,
                                              const RowTab *rows, const ColTab *cols,
                                              const float *dh, int ld, int r0, int r1, int c0,
-                                             int c1) {
+                                             int c1, int *skew = nullptr) {
   const int lane = threadIdx.x & 63;
+  if (skew) *skew = 0;
   // DEM samples blended by the tile's pixels (the index is monotone in the
   // row / column; one sample of slack on each side)
   const int ya = dem_index(rows[r0].fy, dem.nrow), yb = dem_index(rows[r1].fy, dem.nrow);
@@ -670,6 +678,16 @@ __device__ __forceinline__ bool tile_outside(const GeomDev &g, const OffPoly &op
              (q & 4) ? (double)hi + (double)dhi : (double)lo + (double)dlo, X);
   double t, rng, rg, az, drg, daz;
   const bool ok = geo_position(g, op, X, t, rng, rg, az, drg, daz) < 52;
+  if (skew) {
+    // corners q = 0 (r0, c0) and q = 2 (r1, c0), lowest height
+    const double ra = wave_read(rg, 0), aa = wave_read(az, 0);
+    const double rb = wave_read(rg, 2), ab = wave_read(az, 2);
+    const bool oka = __builtin_amdgcn_readlane((int)ok, 0), okb = __builtin_amdgcn_readlane((int)ok, 2);
+    if (oka && okb && fabs(ab - aa) >= 1.0) {
+      const double k = rint((rb - ra) / (ab - aa));
+      *skew = (int)fmin(fmax(k, (double)-kSkewMax), (double)kSkewMax);
+    }
+  }
   const double lr = INTERP == S1IFG_INTERP_BILINEAR ? g.fvs : g.fvs + 3;
   const double ur = INTERP == S1IFG_INTERP_BILINEAR ? g.lvs - 1 : g.lvs - 4;
   const double la = g.flv + 5, ua = g.llv - 5, M = kSkipMargin;

@@ -12,8 +12,12 @@
 // sampling positions and sin/cos of the phase in registers; the block
 // reduces the bounding box of the 8 x 8 tap windows, loads that window of
 // the deramped burst into LDS (stage_window: many loads in flight per
-// thread), and every pixel is interpolated from LDS (a window larger than
-// kPairWin samples is read from global memory instead).  All ways run the
+// thread), and every pixel is interpolated from LDS.  A window larger than
+// kPairWin samples is read from global memory instead; on raw bursts it is
+// first sheared by the tile's skew (row a of the window starts sk columns
+// after row a - 1: a grid column runs diagonally through the burst, so its
+// tap rows shift in range) and staged if that fits, since a raw burst's
+// taps read from global memory are deramped one by one.  All ways run the
 // same arithmetic in the same order as pair_ml_pixel, so the products are
 // bit-identical to it.
 #pragma once
@@ -33,7 +37,9 @@ constexpr int kStage = 16;           // window loads in flight per thread while 
 struct PairLds {
   float2 win[kPairWin];  // one burst's tap window at a time
   int box[2][4];         // per burst: min intr, min inta, max intr, max inta
+  int sbox[2][2];        // per burst: first and last column of the sheared window
   int out[2];            // per burst: every pixel of the tile outside the burst
+  int skew[2];           // per burst: shear of the window (tile_outside)
 };
 
 // Stage rows [wa0, wa0 + H) x columns [wr0, wr0 + W) of burst z (row
@@ -94,6 +100,57 @@ __device__ __forceinline__ void stage_window_deramp(const float2 *__restrict__ z
   }
 }
 
+// Floor and ceiling of a / b for b > 0.
+__device__ __forceinline__ int floor_div(int a, int b) { return a >= 0 ? a / b : -((b - 1 - a) / b); }
+__device__ __forceinline__ int ceil_div(int a, int b) { return -floor_div(-a, b); }
+
+// stage_window_deramp for a sheared window (row r holds burst columns
+// [wr0 + r dk, wr0 + r dk + W), dk != 0): the lanes walk burst columns, so
+// that each keeps the deramp_seq of one column; a column meets the rows r
+// with 0 <= col - wr0 - r dk < W, of which wavefront w takes r = w mod 4.
+// Columns outside the burst (never tapped) are not staged.
+__device__ __forceinline__ void stage_window_deramp_skew(const float2 *__restrict__ z,
+                                                         const GeomDev &g, int wa0, int wr0,
+                                                         int W, int H, int dk, float2 *win) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ns = g.samples;
+  const double eta0 = -g.lines * g.ati / 2.;
+  const int gc0 = wr0 + min(0, dk * (H - 1));
+  const int span = W + abs(dk) * (H - 1);
+  for (int c = lane; c < span; c += 64) {
+    const int col = gc0 + c;
+    if ((unsigned)col >= (unsigned)ns) continue;
+    const int e = col - wr0;  // the column sits at e - r dk in row r
+    int rlo, rhi;
+    if (dk > 0) {
+      rlo = floor_div(e - W, dk) + 1;
+      rhi = floor_div(e, dk);
+    } else {
+      rlo = ceil_div(-e, -dk);
+      rhi = floor_div(W - 1 - e, -dk);
+    }
+    rlo = max(rlo, 0);
+    rhi = min(rhi, H - 1);
+    const int r0 = rlo + ((wave - rlo) & 3);  // this wavefront's first row
+    if (r0 > rhi) continue;
+    DerampSeq q =
+        deramp_seq(eta0 + (wa0 + r0) * g.ati, 4 * g.ati, -0.5 * g.kt[col], g.etaref[col]);
+    for (int rb = r0; rb <= rhi; rb += 4 * kStage) {
+      float2 v[kStage];
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) {
+        const int r = rb + 4 * k;
+        if (r <= rhi) v[k] = z[(size_t)(wa0 + r) * ns + col];
+      }
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) {
+        const int r = rb + 4 * k;
+        if (r <= rhi) win[r * W + (e - r * dk)] = deramp_next(q, v[k]);
+      }
+    }
+  }
+}
+
 // sinc_interp_raw out of line, for the rare tiles whose window exceeds the
 // LDS budget (one copy shared by every call site).
 __device__ __attribute__((noinline)) float2 sinc_interp_raw_global(const GeomDev &g,
@@ -115,8 +172,10 @@ __device__ __forceinline__ int wave_max_i(int v) {
 // [0] tiles, [1] burst-tiles skipped by the corner test, [2] burst-tiles
 // interpolated from the LDS window, [3] burst-tiles read from global memory
 // (window larger than kPairWin), [4] burst-tiles not skipped whose pixels
-// all lie outside the validity window.
-enum { kStatTiles, kStatSkipped, kStatStaged, kStatGlobal, kStatEmpty, kStatCount };
+// all lie outside the validity window, [5] raw burst-tiles interpolated
+// from a sheared LDS window.
+enum { kStatTiles, kStatSkipped, kStatStaged, kStatGlobal, kStatEmpty, kStatSheared, kStatCount };
+static_assert(kStatCount == S1IFG_TILE_STATS, "tile classes of s1ifg_ctx_set_tile_stats");
 
 // One tile (tile_y, tile_x) of look cells.  All 256 threads of the block
 // call it (block barriers); rl * cl <= kPairMaxPix.  A block that runs
@@ -137,19 +196,26 @@ __device__ __forceinline__ void pair_tile_sinc(
   const bool active = cm < nc_ml && rm < nr_ml;
   const int np = NP ? NP : rl * cl;
   if (tid < 8) L.box[tid >> 2][tid & 3] = (tid & 3) < 2 ? INT_MAX : INT_MIN;
+  if (tid < 4) L.sbox[tid >> 1][tid & 1] = (tid & 1) ? INT_MIN : INT_MAX;
   // wavefront b < 2: is the whole tile outside burst b?
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (wave < 2) {
     const int nrt = min(nr_ml - tile_y * kPairTile, kPairTile);
     const int nct = min(nc_ml - tile_x * kPairTile, kPairTile);
     const int r0 = tile_y * kPairTile * rl, c0 = tile_x * kPairTile * cl;
+    int sk = 0;
     const bool o = tile_outside<S1IFG_INTERP_SINC>(wave ? G1 : G0, wave ? op1 : none, dem, rows,
                                                    cols, dh, gr.ncol, r0, r0 + nrt * rl - 1, c0,
-                                                   c0 + nct * cl - 1);
-    if ((tid & 63) == 0) L.out[wave] = o;
+                                                   c0 + nct * cl - 1, RAW ? &sk : nullptr);
+    if ((tid & 63) == 0) {
+      L.out[wave] = o;
+      L.skew[wave] = sk;
+    }
   }
-  __syncthreads();  // box initialised, L.out set
+  __syncthreads();  // boxes initialised, L.out and L.skew set
   const bool out0 = L.out[0], out1 = L.out[1];
+  const int sk0 = __builtin_amdgcn_readfirstlane(L.skew[0]);
+  const int sk1 = __builtin_amdgcn_readfirstlane(L.skew[1]);
   if (stats && tid == 0) {
     atomicAdd(&stats[kStatTiles], 1ull);
     atomicAdd(&stats[kStatSkipped], (unsigned long long)(out0 + out1));
@@ -228,11 +294,47 @@ __device__ __forceinline__ void pair_tile_sinc(
     const int mr = L.box[0][0], ma = L.box[0][1], xr = L.box[0][2], xa = L.box[0][3];
     const bool any = mr <= xr;
     const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
-    const bool staged = any && W * H <= kPairWin;
-    const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
+    const int wa0 = ma - 3;
+    // the rectangular window; on raw bursts, when it exceeds kPairWin, the
+    // window sheared by the tile's skew (row a holds columns lo + a sk .. hi
+    // + a sk, lo and hi the extremes of the taps' columns less their rows'
+    // shear)
+    bool staged = any && W * H <= kPairWin;
+    int wc = (int)W, wr0 = mr - 3, dk = 0;
+    if (RAW && any && !staged && sk0 != 0) {  // block-uniform
+      int lo_s = INT_MAX, hi_s = INT_MIN;
+#pragma unroll
+      for (int p = 0; p < kPairMaxPix; ++p) {
+        if (vmask & (1u << p)) {
+          const int ta = (ia[p] - 3) * sk0, tb = (ia[p] + 4) * sk0;
+          lo_s = min(lo_s, ir[p] - 3 - max(ta, tb));
+          hi_s = max(hi_s, ir[p] + 4 - min(ta, tb));
+        }
+      }
+      const int w4 = wave_min_i(lo_s), w5 = wave_max_i(hi_s);
+      if ((tid & 63) == 0) {
+        atomicMin(&L.sbox[0][0], w4);
+        atomicMax(&L.sbox[0][1], w5);
+      }
+      __syncthreads();
+      const long S = (long)L.sbox[0][1] - L.sbox[0][0] + 1;
+      if (S * H <= kPairWin) {
+        staged = true;
+        dk = sk0;
+        wc = (int)S;
+        wr0 = L.sbox[0][0] + wa0 * dk;
+      }
+    }
+    const int ld = wc - dk;  // LDS index step of a tap row
     if (stats && tid == 0 && !out0)
-      atomicAdd(&stats[staged ? kStatStaged : any ? kStatGlobal : kStatEmpty], 1ull);
-    if (staged && RAW) stage_window_deramp(z0, G0, wa0, wr0, wc, (int)H, L.win);
+      atomicAdd(&stats[staged ? (dk ? kStatSheared : kStatStaged) : any ? kStatGlobal : kStatEmpty],
+                1ull);
+    if (staged && RAW) {
+      if (dk)
+        stage_window_deramp_skew(z0, G0, wa0, wr0, wc, (int)H, dk, L.win);
+      else
+        stage_window_deramp(z0, G0, wa0, wr0, wc, (int)H, L.win);
+    }
     if (staged && !RAW) stage_window(z0, G0.samples, wa0, wr0, wc, (int)H, L.win);
     __syncthreads();
 #pragma unroll
@@ -240,7 +342,7 @@ __device__ __forceinline__ void pair_tile_sinc(
       v0[p] = make_float2(0.f, 0.f);
       if (p < np && (vmask & (1u << p))) {
         if (staged)
-          v0[p] = sinc_interp(&L.win[(ia[p] - 3 - wa0) * wc + (ir[p] - 3 - wr0)], wc, fr[p],
+          v0[p] = sinc_interp(&L.win[(ia[p] - 3 - wa0) * ld + (ir[p] - 3 - wr0)], ld, fr[p],
                               fa[p], sn[p], cs[p]);
         else if (RAW)
           v0[p] = sinc_interp_raw_global(G0, z0, ia[p], ir[p], fr[p], fa[p], sn[p], cs[p]);
@@ -306,11 +408,47 @@ __device__ __forceinline__ void pair_tile_sinc(
     const int mr = L.box[1][0], ma = L.box[1][1], xr = L.box[1][2], xa = L.box[1][3];
     const bool any = mr <= xr;
     const long W = any ? (long)xr - mr + 8 : 0, H = any ? (long)xa - ma + 8 : 0;
-    const bool staged = any && W * H <= kPairWin;
-    const int wc = (int)W, wr0 = mr - 3, wa0 = ma - 3;
+    const int wa0 = ma - 3;
+    // the rectangular window; on raw bursts, when it exceeds kPairWin, the
+    // window sheared by the tile's skew (row a holds columns lo + a sk .. hi
+    // + a sk, lo and hi the extremes of the taps' columns less their rows'
+    // shear)
+    bool staged = any && W * H <= kPairWin;
+    int wc = (int)W, wr0 = mr - 3, dk = 0;
+    if (RAW && any && !staged && sk1 != 0) {  // block-uniform
+      int lo_s = INT_MAX, hi_s = INT_MIN;
+#pragma unroll
+      for (int p = 0; p < kPairMaxPix; ++p) {
+        if (vmask & (1u << p)) {
+          const int ta = (ia[p] - 3) * sk1, tb = (ia[p] + 4) * sk1;
+          lo_s = min(lo_s, ir[p] - 3 - max(ta, tb));
+          hi_s = max(hi_s, ir[p] + 4 - min(ta, tb));
+        }
+      }
+      const int w4 = wave_min_i(lo_s), w5 = wave_max_i(hi_s);
+      if ((tid & 63) == 0) {
+        atomicMin(&L.sbox[1][0], w4);
+        atomicMax(&L.sbox[1][1], w5);
+      }
+      __syncthreads();
+      const long S = (long)L.sbox[1][1] - L.sbox[1][0] + 1;
+      if (S * H <= kPairWin) {
+        staged = true;
+        dk = sk1;
+        wc = (int)S;
+        wr0 = L.sbox[1][0] + wa0 * dk;
+      }
+    }
+    const int ld = wc - dk;  // LDS index step of a tap row
     if (stats && tid == 0 && !out1)
-      atomicAdd(&stats[staged ? kStatStaged : any ? kStatGlobal : kStatEmpty], 1ull);
-    if (staged && RAW) stage_window_deramp(z1, G1, wa0, wr0, wc, (int)H, L.win);
+      atomicAdd(&stats[staged ? (dk ? kStatSheared : kStatStaged) : any ? kStatGlobal : kStatEmpty],
+                1ull);
+    if (staged && RAW) {
+      if (dk)
+        stage_window_deramp_skew(z1, G1, wa0, wr0, wc, (int)H, dk, L.win);
+      else
+        stage_window_deramp(z1, G1, wa0, wr0, wc, (int)H, L.win);
+    }
     if (staged && !RAW) stage_window(z1, G1.samples, wa0, wr0, wc, (int)H, L.win);
     __syncthreads();
 #pragma unroll
@@ -318,7 +456,7 @@ __device__ __forceinline__ void pair_tile_sinc(
       v1[p] = make_float2(0.f, 0.f);
       if (p < np && (vmask & (1u << p))) {
         if (staged)
-          v1[p] = sinc_interp(&L.win[(ia[p] - 3 - wa0) * wc + (ir[p] - 3 - wr0)], wc, fr[p],
+          v1[p] = sinc_interp(&L.win[(ia[p] - 3 - wa0) * ld + (ir[p] - 3 - wr0)], ld, fr[p],
                               fa[p], sn[p], cs[p]);
         else if (RAW)
           v1[p] = sinc_interp_raw_global(G1, z1, ia[p], ir[p], fr[p], fa[p], sn[p], cs[p]);

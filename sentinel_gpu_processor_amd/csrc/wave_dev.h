@@ -58,4 +58,13 @@ __device__ __forceinline__ T wave_reduce(T v, Op op) {
   return v;
 }
 
+// Lane l's value of a double (wavefront-uniform result; l a constant lane
+// index).
+__device__ __forceinline__ double wave_read(double v, int l) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 }  // namespace s1ifg

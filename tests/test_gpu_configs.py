@@ -127,13 +127,14 @@ C2_CROP = (0, 0, 1500, 3072)  # rows, columns: the grid's west edge, outside cor
 def test_c2_tile_classes(c2):
     """At C2 the pair kernel meets every tile class: skipped (outside a
     burst), tap window staged in LDS, window beyond the LDS budget (global
-    reads)."""
-    tiles, skipped, staged, glob, empty = c2["stats"]
+    reads; only raw bursts shear such windows)."""
+    tiles, skipped, staged, glob, empty, sheared = c2["stats"]
     print(f"c2 tiles {tiles}: burst-tiles skipped {skipped}, staged {staged}, global {glob}, "
           f"empty {empty}")
     grid = c2["sc"]["grid"]
     assert tiles == -(-grid["nrow"] // 16) * -(-(grid["ncol"] // 4) // 16)
     assert skipped + staged + glob + empty == 2 * tiles
+    assert sheared == 0
     assert skipped > 0.2 * 2 * tiles
     assert staged > 0.05 * 2 * tiles
     assert glob > 0
@@ -271,14 +272,31 @@ def test_c2_full_size_physics(ctx):
 
 
 def test_c2_raw_pair_matches_deramped(ctx, c2):
-    """The raw-burst pair (samples deramped as staged, and per tap on the
-    tiles read from global memory) over the whole C2 pair against deramp +
-    the pair kernel, and against the oracle on the crop."""
+    """The raw-burst pair (samples deramped as staged; a window beyond the
+    LDS budget sheared along the tile's skew and staged when that fits, its
+    samples deramped per burst column; per tap on the tiles still read from
+    global memory) over the whole C2 pair against deramp + the pair kernel,
+    and against the oracle on the crop.  Most of the windows the deramped
+    kernel reads from global memory fit once sheared."""
     from sentinel_gpu_processor_amd import kernels as K
+    from sentinel_gpu_processor_amd import _native as N
     sc = c2["sc"]
-    ifg, pr, ps = K.pair(ctx, c2["gr"], c2["gs"], sc["grid"], c2["dem_t"], sc["dem"],
-                         sc["ref_slc"], sc["sec_slc"], "sinc", 1, 4, offpoly=sc["offpoly"],
-                         raw=True)
+    stats = torch.zeros(N.TILE_STATS, dtype=torch.int64, device="cuda:0")
+    ctx.set_tile_stats(stats)
+    try:
+        ifg, pr, ps = K.pair(ctx, c2["gr"], c2["gs"], sc["grid"], c2["dem_t"], sc["dem"],
+                             sc["ref_slc"], sc["sec_slc"], "sinc", 1, 4, offpoly=sc["offpoly"],
+                             raw=True)
+        st = _np(stats).tolist()
+    finally:
+        ctx.set_tile_stats(None)
+    tiles, skipped, staged, glob, empty, sheared = st
+    print(f"c2 raw tiles {tiles}: burst-tiles skipped {skipped}, staged {staged}, "
+          f"sheared {sheared}, global {glob}, empty {empty}")
+    assert skipped + staged + sheared + glob + empty == 2 * tiles
+    assert (skipped, staged + sheared + glob, empty) == (
+        c2["stats"][1], c2["stats"][2] + c2["stats"][3], c2["stats"][4])
+    assert sheared > 0 and glob < 0.2 * c2["stats"][3]
     d = (ifg - c2["ifg"]).abs().max().item()
     assert d <= 2e-5 * c2["ifg"].abs().max().item()
     assert bool(((pr > 0) == (c2["pr"] > 0)).all())

@@ -222,18 +222,22 @@ def test_pair_raw_matches_oracle(ctx, interp, name):
     assert np.max(np.abs(pr - p2)) <= 2e-5 * p2.max()
 
 
+@pytest.mark.parametrize("coarse", [(8, 4), (2, 1)])
 @pytest.mark.parametrize("raw", [False, True])
-def test_pair_coarse_grid_global_taps(ctx, raw):
+def test_pair_coarse_grid_global_taps(ctx, raw, coarse):
     """A target grid 8 x 4 times coarser than the bursts' sampling: one
     wavefront's rows of a tile span a tap window beyond the LDS budget, so
-    the taps are read from global memory (deramped per tap when raw); the
-    products still match the oracle on that grid."""
+    the taps are read from global memory (deramped per tap when raw).  2 x 1
+    times coarser: on raw bursts some of those windows fit the budget once
+    sheared along the tile's skew (staged sheared, deramped per burst column
+    as staged).  The products still match the oracle on that grid."""
     from sentinel_gpu_processor_amd import _native as N
     from sentinel_gpu_processor_amd import kernels as K
     from tests.configs import oracle_pair_crop
     sc = scene("small_misreg")
-    g = dict(sc["grid"], nrow=sc["grid"]["nrow"] // 8, ncol=sc["grid"]["ncol"] // 4,
-             dlat=sc["grid"]["dlat"] * 8, dlon=sc["grid"]["dlon"] * 4)
+    fr, fc = coarse
+    g = dict(sc["grid"], nrow=sc["grid"]["nrow"] // fr, ncol=sc["grid"]["ncol"] // fc,
+             dlat=sc["grid"]["dlat"] * fr, dlon=sc["grid"]["dlon"] * fc)
     gr, gs = _geoms(ctx, sc)
     dem = sc["dem"]
     ref, sec = _dev(sc["ref_slc"]), _dev(sc["sec_slc"])
@@ -248,6 +252,10 @@ def test_pair_coarse_grid_global_taps(ctx, raw):
     finally:
         ctx.set_tile_stats(None)
     assert st[3] > 0, st
+    if raw and coarse == (2, 1):
+        assert st[5] > 0, st
+    if not raw:
+        assert st[5] == 0, st
     ifg, pr, ps = (_np(t) for t in out)
     oi, o1, o2, _, _ = oracle_pair_crop(sc, sc["ref_slc"], sc["sec_slc"], g, "sinc")
     m = (o1 > 0) & (o2 > 0) & (np.abs(oi) > 1e-3 * np.abs(oi).max())
