@@ -107,8 +107,10 @@ __device__ __forceinline__ void coherence_tile(const float2 *__restrict__ ifg,
 // sums are separable: the strip's input rows are read once, 4 at a time,
 // into LDS; each row's horizontal win-sums go into a ring of rows; each
 // output pixel adds the win row sums above and below it, and f(r, c, sa,
-// sb, s1, s2) gets the pixel's sums.  Contains block barriers (uniform);
-// the last one lets the caller reuse L.
+// sb, s1, s2) gets the pixel's sums.  The loads of the next 4 input rows
+// are issued before the current rows' sums (they are in flight while the
+// block sums).  Contains block barriers (uniform); the last one lets the
+// caller reuse L.
 constexpr int kStripRows = 32, kStripMaxHalo = 6, kStripRing = 16;
 static_assert(kStripRing >= kCohTY + 2 * kStripMaxHalo, "ring holds a window of rows");
 
@@ -127,28 +129,45 @@ __device__ __forceinline__ void window_sums_strip(const float2 *__restrict__ za,
   const int x = threadIdx.x, y = threadIdx.y;
   const int c0 = strip_x * kCohTX, rb = strip_y * kStripRows;
   const int re = min(rb + kStripRows, nr);
-  int nl = rb - h;  // next input row to load
+  // this thread's input columns i = x and x + kCohTX (the latter when
+  // below kCohTX + w2), of input row base + y, into registers
+  static_assert(kCohTX >= 2 * kStripMaxHalo, "two columns per thread cover the halo");
+  float va[2], vb[2], vu[2], vv[2];
+  const auto load_rows = [&](int base) {
+    const int rr = base + y;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int i = x + k * kCohTX, cc = c0 - h + i;
+      va[k] = vb[k] = vu[k] = vv[k] = 0.f;
+      if (i < kCohTX + w2 && rr >= 0 && rr < nr && cc >= 0 && cc < nc) {
+        const size_t o = (size_t)rr * nc + cc;
+        const float2 z = za[o];
+        va[k] = z.x;
+        vb[k] = z.y;
+        vu[k] = zb[o];
+        vv[k] = zc[o];
+      }
+    }
+  };
+  int nl = rb - h;  // next input row to stage
+  // the last input row staged: that of the last output row group
+  const int last = rb + (re - 1 - rb) / kCohTY * kCohTY + kCohTY - 1 + h;
+  load_rows(nl);
   for (int r0 = rb; r0 < re; r0 += kCohTY) {
     // horizontal sums of the input rows up to r0 + kCohTY - 1 + h
     while (nl <= r0 + kCohTY - 1 + h) {
-      const int rr = nl + y;
-      for (int i = x; i < kCohTX + w2; i += kCohTX) {
-        const int cc = c0 - h + i;
-        float a = 0.f, b = 0.f, u = 0.f, v = 0.f;
-        if (rr >= 0 && rr < nr && cc >= 0 && cc < nc) {
-          const size_t o = (size_t)rr * nc + cc;
-          const float2 z = za[o];
-          a = z.x;
-          b = z.y;
-          u = zb[o];
-          v = zc[o];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int i = x + k * kCohTX;
+        if (i < kCohTX + w2) {
+          L.in[0][y][i] = va[k];
+          L.in[1][y][i] = vb[k];
+          L.in[2][y][i] = vu[k];
+          L.in[3][y][i] = vv[k];
         }
-        L.in[0][y][i] = a;
-        L.in[1][y][i] = b;
-        L.in[2][y][i] = u;
-        L.in[3][y][i] = v;
       }
       __syncthreads();
+      if (nl + kCohTY <= last) load_rows(nl + kCohTY);  // in flight during the sums
       float sa = 0.f, sb = 0.f, su = 0.f, sv = 0.f;
       for (int j = 0; j <= w2; ++j) {
         sa += L.in[0][y][x + j];
@@ -156,7 +175,7 @@ __device__ __forceinline__ void window_sums_strip(const float2 *__restrict__ za,
         su += L.in[2][y][x + j];
         sv += L.in[3][y][x + j];
       }
-      const int slot = rr & (kStripRing - 1);
+      const int slot = (nl + y) & (kStripRing - 1);
       L.hs[0][slot][x] = sa;
       L.hs[1][slot][x] = sb;
       L.hs[2][slot][x] = su;
