@@ -24,6 +24,8 @@
 // filter nor partial traffic.
 #pragma once
 
+#include <climits>
+
 #include "gold_dev.h"
 #include "gold_mfma_dev.h"
 
@@ -54,6 +56,11 @@ __host__ __device__ __forceinline__ long band_items(int pi_n, int pj_n) {
 }
 __host__ __device__ __forceinline__ size_t band_partial_elems(int pi_n, int pj_n) {
   return (size_t)band_items(pi_n, pj_n) * kBandH * kBandCols;
+}
+// The band sum addresses the partial sums with 32-bit offsets (about 3.4
+// elements per image pixel: images up to ~630 million pixels).
+__host__ __device__ __forceinline__ bool band_offsets_fit(int pi_n, int pj_n) {
+  return band_partial_elems(pi_n, pj_n) <= (size_t)INT_MAX;
 }
 // A pair's Goldstein buffer: the partial sums, then (256-byte aligned) one
 // int per item, its zero mark: the item's partial columns below it (from
@@ -195,8 +202,10 @@ static_assert(kBandCW >= 5, "a pixel's patch columns must span at most 2 chunks"
 struct BandSumLds {
   float cx[kSumTX], sx[kSumTX];
   int nch[kSumTX], ch[kSumTX][2], xo[kSumTX][2];
+  int cb[kSumTX][2];  // column part of a partial's offset: chunk * kBandH * kBandCols + xo
   float cy[kSumTY], sy[kSumTY];
   int nb[kSumTY], grp[kSumTY][kSumGroups], ty[kSumTY][kSumGroups];
+  int rb[kSumTY][kSumGroups];  // row part: (group * nchunk * kBandH + ty) * kBandCols
   // zero marks of the items the tile meets: groups from the first row's
   // first, chunks from the first column's first (kSumTY rows advance the
   // groups by at most one; kSumTX columns meet at most 3 chunks)
@@ -235,6 +244,7 @@ __device__ __forceinline__ void band_sum_tile(const float2 *__restrict__ partial
         if (x < a || x >= b) continue;
         L.ch[tx][n] = c;
         L.xo[tx][n] = x - a;
+        L.cb[tx][n] = c * (kBandH * kBandCols) + (x - a);
         ++n;
       }
     }
@@ -260,6 +270,7 @@ __device__ __forceinline__ void band_sum_tile(const float2 *__restrict__ partial
         if ((n == 0 || L.grp[tx][n - 1] != gp) && n < kSumGroups) {
           L.grp[tx][n] = gp;
           L.ty[tx][n] = yy - gold_origin(gp * kBandRows, nrow);
+          L.rb[tx][n] = (gp * nchunk * kBandH + L.ty[tx][n]) * kBandCols;
           ++n;
         }
       }
@@ -287,23 +298,23 @@ __device__ __forceinline__ void band_sum_tile(const float2 *__restrict__ partial
     float2 r = make_float2(0.f, 0.f);
     if (ws > 0.f) {
       // all covering partials loaded first, then added in (group, chunk)
-      // order
-      const int nb = L.nb[tyi], nch = L.nch[tx];
+      // order; a wavefront is one tile row, so the row terms are uniform
+      const int nb = __builtin_amdgcn_readfirstlane(L.nb[tyi]), nch = L.nch[tx];
       float2 v[kSumGroups][2];
 #pragma unroll
-      for (int i = 0; i < kSumGroups; ++i)
+      for (int i = 0; i < kSumGroups; ++i) {
+        const int rbi = __builtin_amdgcn_readfirstlane(L.rb[tyi][i]);
+        const int gi = min(max(__builtin_amdgcn_readfirstlane(L.grp[tyi][i]) - g0, 0), kSumGroups);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
           v[i][k] = make_float2(0.f, 0.f);
           if (i < nb && k < nch) {
             // a column below the item's zero mark holds zeros, not written
-            const int gi = min(max(L.grp[tyi][i] - g0, 0), kSumGroups);
             const int ci = min(max(L.ch[tx][k] - c0, 0), 2);
-            if (L.xo[tx][k] >= L.zb[gi][ci])
-              v[i][k] = partial[(((size_t)L.grp[tyi][i] * nchunk + L.ch[tx][k]) * kBandH +
-                                 L.ty[tyi][i]) * kBandCols + L.xo[tx][k]];
+            if (L.xo[tx][k] >= L.zb[gi][ci]) v[i][k] = partial[rbi + L.cb[tx][k]];
           }
         }
+      }
       float ax = 0.f, ay = 0.f;
 #pragma unroll
       for (int i = 0; i < kSumGroups; ++i)
@@ -313,7 +324,8 @@ __device__ __forceinline__ void band_sum_tile(const float2 *__restrict__ partial
             ax += v[i][k].x;
             ay += v[i][k].y;
           }
-      r = make_float2(ax / ws, ay / ws);
+      const float inv = 1.0f / ws;
+      r = make_float2(ax * inv, ay * inv);
     }
     out[(size_t)y * ncol + x] = r;
   }

@@ -542,6 +542,9 @@ int s1ifg_batch_create(s1ifg_ctx *ctx, const s1ifg_pair_task *tasks, int n,
                          p, d.nr_ml, d.nc_ml, kGoldN);
       if (d.filt == d.ifg) return set_error(S1IFG_EINVAL, "pair %d: in-place filtering is not supported", p);
       goldstein_grid(d.nr_ml, d.nc_ml, &d.pi_n, &d.pj_n);
+      if (o.gold_mode == S1IFG_GOLD_FP16_SPLIT && !band_offsets_fit(d.pi_n, d.pj_n))
+        return set_error(S1IFG_ERANGE, "pair %d: multilooked image %d x %d too large for the fp16_split band sums",
+                         p, d.nr_ml, d.nc_ml);
       patch_off[p] = patch_bytes;
       patch_bytes += align256(o.gold_mode == S1IFG_GOLD_FP16_SPLIT
                                   ? band_buffer_bytes(d.pi_n, d.pj_n)

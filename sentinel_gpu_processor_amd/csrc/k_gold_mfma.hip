@@ -57,6 +57,8 @@ size_t band_workspace_bytes(int pi_n, int pj_n) {
   return band_counter_offset(pi_n, pj_n) + 256;
 }
 
+bool band_fits(int pi_n, int pj_n) { return band_offsets_fit(pi_n, pj_n); }
+
 int goldstein_band(s1ifg_ctx *ctx, const float2 *in, int nrow, int ncol, float alpha, int pi_n,
                    int pj_n, void *ws, float2 *out) {
   float2 *partial = (float2 *)ws;

@@ -60,6 +60,7 @@ int goldstein_grid(int nrow, int ncol, int *pi_n, int *pj_n) {
 }
 
 size_t band_workspace_bytes(int pi_n, int pj_n);
+bool band_fits(int pi_n, int pj_n);
 int goldstein_band(s1ifg_ctx *ctx, const float2 *in, int nrow, int ncol, float alpha, int pi_n,
                    int pj_n, void *ws, float2 *out);
 
@@ -83,6 +84,9 @@ extern "C" int s1ifg_goldstein(s1ifg_ctx *ctx, const s1ifg_cf32 *in, int nrow, i
   size_t npatch = (size_t)pi_n * pj_n;
   void *ws = nullptr;
   if (mode == S1IFG_GOLD_FP16_SPLIT) {
+    if (!band_fits(pi_n, pj_n))
+      return set_error(S1IFG_ERANGE, "image %d x %d too large for the fp16_split band sums", nrow,
+                       ncol);
     if ((rc = ctx_workspace(ctx, band_workspace_bytes(pi_n, pj_n), &ws))) return rc;
     return goldstein_band(ctx, (const float2 *)in, nrow, ncol, alpha, pi_n, pj_n, ws,
                           (float2 *)out);
