@@ -130,6 +130,47 @@ def test_geo2rdr_curved_orbit(ctx):
     assert np.max(np.abs(_np(rng) - o["rng"])) < 5e-6
 
 
+def _bent(orbit, A, B, C):
+    t, pos, vel = orbit
+    e = t - t.mean()
+    p2 = pos + A * e[:, None] ** 2 + B * e[:, None] ** 3 + C * e[:, None] ** 4
+    v2 = vel + 2 * A * e[:, None] + 3 * B * e[:, None] ** 2 + 4 * C * e[:, None] ** 3
+    return (t, p2, v2)
+
+
+@pytest.mark.parametrize("raw", [False, True])
+def test_pair_curved_orbits(ctx, raw):
+    """The fused pair kernel on bent orbits (about 9 m/s^2 of acceleration,
+    as a circular orbit's, plus cubic and quartic terms): each look cell's
+    first pixel starts its Newton solve at the time the tile's corner
+    solutions predict, the other pixels at the first pixel's root; the
+    products equal the oracle's, whose Newton iteration starts at the burst
+    mid-time."""
+    from sentinel_gpu_processor_amd import _native as N
+    from sentinel_gpu_processor_amd import kernels as K
+    from tests.configs import oracle_pair_crop
+    sc = dict(scene("small_misreg"))
+    sc["orbit_ref"] = _bent(sc["orbit_ref"], np.array([3.5, -2.0, -2.5]),
+                            np.array([2e-3, 1e-3, -3e-3]), np.array([1e-4, -2e-4, 1e-4]))
+    sc["orbit_sec"] = _bent(sc["orbit_sec"], np.array([3.4, -2.1, -2.5]),
+                            np.array([-1e-3, 2e-3, -2e-3]), np.array([-1e-4, 1e-4, 2e-4]))
+    gr = N.Geometry(ctx, sc["orbit_ref"], sc["burst_ref"])
+    gs = N.Geometry(ctx, sc["orbit_sec"], sc["burst_sec"])
+    dem = sc["dem"]
+    ref, sec = _dev(sc["ref_slc"]), _dev(sc["sec_slc"])
+    if not raw:
+        ref, sec = K.deramp(ctx, gr, ref), K.deramp(ctx, gs, sec)
+    ifg, pr, ps = (_np(t) for t in K.pair(ctx, gr, gs, sc["grid"], _dev(dem["data"]), dem, ref,
+                                         sec, "sinc", 1, 4, offpoly=sc["offpoly"], raw=raw))
+    oi, o1, o2, _, _ = oracle_pair_crop(sc, sc["ref_slc"], sc["sec_slc"], sc["grid"], "sinc")
+    m = (o1 > 0) & (o2 > 0) & (np.abs(oi) > 1e-3 * np.abs(oi).max())
+    assert m.mean() > 0.2
+    assert phase_rms(ifg, oi, m) < 1e-4
+    assert np.max(np.abs(pr[m] - o1[m]) / o1[m]) < 1e-4
+    assert np.count_nonzero((pr > 0) != (o1 > 0)) <= 2
+    assert np.count_nonzero((ps > 0) != (o2 > 0)) <= 2
+
+
 @pytest.mark.parametrize("interp", ["bilinear", "sinc"])
 @pytest.mark.parametrize("name", ["small", "small_misreg"])
 def test_geocode_matches_oracle(ctx, interp, name):
