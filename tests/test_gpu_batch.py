@@ -49,12 +49,12 @@ def pairs(ctx):
     return out
 
 
-def _single(ctx, p, interp, gold_mode, win):
+def _single(ctx, p, interp, gold_mode, win, rl=1, cl=4):
     from sentinel_gpu_processor_amd import kernels as K
     dr = K.deramp(ctx, p["g_ref"], p["ref_slc"])
     ds = K.deramp(ctx, p["g_sec"], p["sec_slc"])
     ifg, pr, ps = K.pair(ctx, p["g_ref"], p["g_sec"], p["grid"], p["dem_t"], p["dem"], dr, ds,
-                         interp, 1, 4, offpoly=p["offpoly"])
+                         interp, rl, cl, offpoly=p["offpoly"])
     filt = K.goldstein(ctx, ifg, alpha=ALPHA, mode=gold_mode)
     coh = K.coherence(ctx, ifg, pr, ps, win=win, fill="hash", seed=SEED)
     return dict(ifg=ifg, pow_ref=pr, pow_sec=ps, filt=filt, coh=coh)
@@ -67,14 +67,15 @@ def _assert_same(batch, want, what):
             assert np.array_equal(a, b), f"{what}: pair {k} {key} differs in {np.count_nonzero(a != b)} values"
 
 
-@pytest.mark.parametrize("interp,gold_mode,win", [("sinc", "fp16_split", 5),
-                                                  ("bilinear", "fp32", 1),
-                                                  ("sinc", "fp32", 3)])
-def test_batch_bit_identical_to_single_pairs(ctx, pairs, interp, gold_mode, win):
+@pytest.mark.parametrize("interp,gold_mode,win,rl,cl", [("sinc", "fp16_split", 5, 1, 4),
+                                                        ("bilinear", "fp32", 1, 1, 4),
+                                                        ("sinc", "fp32", 3, 1, 4),
+                                                        ("sinc", "fp16_split", 5, 2, 2)])
+def test_batch_bit_identical_to_single_pairs(ctx, pairs, interp, gold_mode, win, rl, cl):
     from sentinel_gpu_processor_amd.batch import PairBatch
-    print(f"batch parity {interp} {gold_mode} win={win}", flush=True)
-    want = [_single(ctx, p, interp, gold_mode, win) for p in pairs]
-    b = PairBatch(ctx, pairs, interp=interp, rl=1, cl=4, alpha=ALPHA, gold_mode=gold_mode,
+    print(f"batch parity {interp} {gold_mode} win={win} looks {rl} x {cl}", flush=True)
+    want = [_single(ctx, p, interp, gold_mode, win, rl, cl) for p in pairs]
+    b = PairBatch(ctx, pairs, interp=interp, rl=rl, cl=cl, alpha=ALPHA, gold_mode=gold_mode,
                   coh_win=win, coh_fill="hash", seed=SEED)
     b.run(graph=False)
     _assert_same(b, want, "eager")

@@ -171,6 +171,36 @@ def test_pair_curved_orbits(ctx, raw):
     assert np.count_nonzero((ps > 0) != (o2 > 0)) <= 2
 
 
+@pytest.mark.parametrize("raw", [False, True])
+@pytest.mark.parametrize("interp,rl,cl", [("sinc", 2, 2), ("sinc", 1, 1), ("sinc", 4, 1),
+                                          ("sinc", 1, 3), ("sinc", 2, 3), ("bilinear", 2, 2)])
+def test_pair_look_shapes(ctx, raw, interp, rl, cl):
+    """Look shapes other than 1 x 4: the tile kernel with the look shape
+    at run time (rl * cl <= 4, sinc), the per-pixel kernel (rl * cl > 4,
+    and bilinear), deramped and raw bursts, against the oracle's looks."""
+    from sentinel_gpu_processor_amd import _native as N
+    from sentinel_gpu_processor_amd import kernels as K
+    from tests.configs import oracle_pair_crop
+    sc = scene("small_misreg")
+    gr, gs = _geoms(ctx, sc)
+    dem = sc["dem"]
+    ref, sec = _dev(sc["ref_slc"]), _dev(sc["sec_slc"])
+    if not raw:
+        ref, sec = K.deramp(ctx, gr, ref), K.deramp(ctx, gs, sec)
+    print(f"pair looks {rl} x {cl} {interp} raw={raw}", flush=True)
+    ifg, pr, ps = (_np(t) for t in K.pair(ctx, gr, gs, sc["grid"], _dev(dem["data"]), dem, ref,
+                                         sec, interp, rl, cl, offpoly=sc["offpoly"], raw=raw))
+    oi, o1, o2, _, _ = oracle_pair_crop(sc, sc["ref_slc"], sc["sec_slc"], sc["grid"], interp,
+                                        rl, cl)
+    assert ifg.shape == oi.shape
+    m = (o1 > 0) & (o2 > 0) & (np.abs(oi) > 1e-3 * np.abs(oi).max())
+    assert m.mean() > 0.2
+    assert phase_rms(ifg, oi, m) < 1e-4
+    assert np.max(np.abs(pr[m] - o1[m]) / o1[m]) < 1e-4
+    assert np.count_nonzero((pr > 0) != (o1 > 0)) <= 2 * rl * cl
+    assert np.count_nonzero((ps > 0) != (o2 > 0)) <= 2 * rl * cl
+
+
 @pytest.mark.parametrize("interp", ["bilinear", "sinc"])
 @pytest.mark.parametrize("name", ["small", "small_misreg"])
 def test_geocode_matches_oracle(ctx, interp, name):
